@@ -1,0 +1,198 @@
+"""Benchmark of the render hot path on one MI355X.
+
+Workload (default): the full plain frame — 854x480, 128 spp per frame, 6
+bounces, NEE, thin lens 35 mm f/2.8 with hexagon bokeh, over a seeded
+250,000-triangle scene (retrocam_amd.workloads.full_frame). One step = one
+frame: reset, then 128 progressive samples in one launch of the path-trace
+kernel (the same arithmetic as 128 Renderer.render passes).
+
+Prints ONE JSON line (rank 0). See DESIGN.md for the byte model behind
+"roofline" and for the CPU baseline sample.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per counted event (reference data layout, per access):
+BYTE_MODEL = {
+    "nodePops": 48,        # BVHNode record per closest-hit pop (pathtrace.comp:155)
+    "childTests": 64,      # two child AABBs per interior expansion (:182-185)
+    "triTests": 48,        # v0, v1, v2 per triangle test (:113-115)
+    "hitUpdates": 52,      # n0, n1, n2 + matIndex per accepted closer hit (:168-171)
+    "shadeHits": 64,       # Material record per shaded hit (:475)
+    "shadowPops": 48,      # BVHNode record per any-hit pop (:220)
+    "shadowTriTests": 48,  # triangle positions per any-hit test (:231)
+    "lightScans": 4,       # CDF weight per scanned light (:347-349)
+    "lightReads": 80,      # selected Light record (:350)
+}
+PIXEL_BYTES_PER_SPP = 40   # accum RGBA32F read+write (32) + variance R32F read+write (8) (:649-653)
+
+
+class HipTimer:
+    """hipEvent pairs recorded on the library's stream (the stream the kernels run on)."""
+
+    def __init__(self, stream):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+        self.stream = C.c_void_p(stream)
+        self.pairs = []
+
+    def _ev(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def start(self):
+        e = self._ev()
+        self.hip.hipEventRecord(e, self.stream)
+        self.pairs.append([e, None])
+
+    def stop(self):
+        e = self._ev()
+        self.hip.hipEventRecord(e, self.stream)
+        self.pairs[-1][1] = e
+
+    def times_ms(self):
+        out = []
+        for a, b in self.pairs:
+            self.hip.hipEventSynchronize(b)
+            ms = C.c_float()
+            self.hip.hipEventElapsedTime(C.byref(ms), a, b)
+            out.append(ms.value)
+            self.hip.hipEventDestroy(a)
+            self.hip.hipEventDestroy(b)
+        self.pairs = []
+        return out
+
+
+def cpu_baseline(w, spp_sample: int):
+    """The oracle (scalar C restatement, OpenMP over rows) on spp_sample samples
+    of the same frame; returns frames/s scaled per spp and Mrays/s."""
+    from oracle import rco
+    o = rco.OracleRenderer(w.scene, w.camera.to_c(), w.params.to_c())
+    t0 = time.perf_counter()
+    for _ in range(spp_sample):
+        o.render_pass(collect_stats=True)
+    dt = time.perf_counter() - t0
+    st = o.stats.as_dict()
+    rays = st["rays"] + st["shadowRays"]
+    fps = 1.0 / (dt / spp_sample * w.spp)
+    return {"value": fps, "unit": "frames/s", "cores": int(rco.lib().rco_num_threads()), "kind": "port",
+            "sample": f"{spp_sample} of {w.spp} spp of the same {w.camera.width}x{w.camera.height} frame "
+                      f"({dt:.1f} s), scaled per spp",
+            "mrays_per_s": rays / dt / 1e6}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=128)
+    ap.add_argument("--cpu-spp", type=int, default=4, help="samples timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the render path runs on a single MI355X "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from retrocam_amd import workloads
+    from retrocam_amd.renderer import COUNTER_FIELDS, RenderContext, Renderer, SceneUploader
+
+    def log(*a):
+        if not args.quiet:
+            print(*a, file=sys.stderr, flush=True)
+
+    w = workloads.full_frame(spp=args.spp)
+    log(f"[bench] scene: {w.scene.nTris} tris, {w.scene.nNodes} nodes, {w.scene.nLights} lights, "
+        f"BVH depth {w.scene.maxDepth}")
+    ctx = RenderContext(0)
+    SceneUploader(ctx).upload(w.scene)
+    r = Renderer(ctx)
+
+    def frame():
+        r.reset()
+        r.render(w.camera, w.params, n_samples=w.spp)
+
+    # Counting pass (separately compiled instrumented kernel, identical results):
+    r.debug_counters(True)
+    frame()
+    ctx.sync()
+    cnt = dict(zip(COUNTER_FIELDS, (int(v) for v in r.debug_counters(False))))
+    npx = w.camera.width * w.camera.height
+    alg_bytes = sum(BYTE_MODEL[k] * cnt[k] for k in BYTE_MODEL) + PIXEL_BYTES_PER_SPP * npx * w.spp
+    rays = cnt["rays"] + cnt["shadowRays"]
+    log(f"[bench] counters per frame: {cnt}")
+
+    for _ in range(args.warmup):
+        frame()
+    ctx.sync()
+    torch.cuda.synchronize()
+    timer = HipTimer(ctx.stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        timer.start()
+        frame()
+        timer.stop()
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = timer.times_ms()
+    ms_per_step = wall / args.steps * 1e3
+    fps = args.steps / wall
+    avg_kernel_s = float(np.mean(kms)) / 1e3
+    achieved = alg_bytes / avg_kernel_s / 1e9
+    log(f"[bench] per-frame event times (ms): {[round(x, 2) for x in kms]}")
+
+    cpu = None
+    if args.cpu_spp > 0:
+        log(f"[bench] CPU baseline: oracle, {args.cpu_spp} spp ...")
+        cpu = cpu_baseline(w, args.cpu_spp)
+
+    out = {
+        "metric": "854x480 frames/s (plain path tracer, 128 spp/frame); Mrays/s",
+        "value": round(fps, 4),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 250k-triangle scene, generated in-process)",
+        "config": {"workload": "full_frame", "width": w.camera.width, "height": w.camera.height,
+                   "spp": w.spp, "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
+                   "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU"},
+        "mrays_per_s": round(rays * fps / 1e6, 2),
+        "rays_per_frame": rays,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "pathtrace_kernel", "alg_bytes_per_launch": int(alg_bytes),
+                     "avg_launch_ms": round(avg_kernel_s * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * retrocam_hip.h — C ABI of libretrocam_hip.so, the MI355X (gfx950) render hot path
+ * of RetroCam: progressive path tracing, SPPM photon trace/insert/gather, the
+ * p00 normalize + a-trous denoise + VHS post chain and the display encode.
+ *
+ * The library replaces the OpenGL work behind the reference's Java classes; the
+ * Java host keeps the scene, the SAH BVH builder, the struct writers and the
+ * settings, and passes the exact float streams those writers produce:
+ *
+ *   GpuTriangle.write  (src/main/java/com/retrocam/scene/GpuTriangle.java:68-79)  32 floats
+ *   BVHNode.write      (src/main/java/com/retrocam/scene/BVHNode.java:32-40)      12 floats
+ *   Material.write     (src/main/java/com/retrocam/scene/Material.java:61-66)     16 floats
+ *   light entries      (src/main/java/com/retrocam/scene/SceneUploader.java:173-205) 20 floats
+ *
+ * Integers inside those streams are stored as float bit patterns
+ * (Float.intBitsToFloat), exactly as the Java writers emit them.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (RC_OK = 0). On failure,
+ *     rc_last_error(ctx) returns a message; nothing aborts the host process.
+ *   - One context owns one HIP stream and all device buffers; calls on one
+ *     context must be serialised by the caller (the reference runs on a single
+ *     GL thread). Separate contexts are independent.
+ *   - Images are row-major with row 0 = the BOTTOM row (GL texture order,
+ *     pathtrace.comp:406-413), pixel (x, y) at index y*width + x.
+ *   - Host arrays passed in are copied before the call returns; the caller may
+ *     free them at once (SceneUploader.java:65-71 frees its buffers after upload).
+ */
+#ifndef RETROCAM_HIP_H
+#define RETROCAM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    RC_OK = 0,
+    RC_ERR_INVALID_ARG = 1,   /* bad pointer, size, index or enum value            */
+    RC_ERR_NO_DEVICE = 2,     /* HIP device missing or not gfx950-capable          */
+    RC_ERR_HIP = 3,           /* a HIP runtime call failed (message has the code)  */
+    RC_ERR_OOM = 4,           /* device allocation failed                          */
+    RC_ERR_STATE = 5,         /* call out of order (e.g. render before upload)     */
+    RC_ERR_LIMIT = 6          /* size beyond a documented library limit            */
+};
+
+/* ---- wire layouts (std430 records as written by the Java host) ---------- */
+typedef struct rc_gpu_triangle {     /* 128 B, GpuTriangle.java:68-79          */
+    float v0[3]; float pad0;
+    float v1[3]; float pad1;
+    float v2[3]; float pad2;
+    float n0[3]; float pad3;
+    float n1[3]; float pad4;
+    float n2[3]; float pad5;
+    float uv0[2]; float uv1[2];
+    float uv2[2]; int32_t matIndex; float pad6;
+} rc_gpu_triangle;
+
+typedef struct rc_bvh_node {         /* 48 B, BVHNode.java:32-40; leaf iff primCount > 0 */
+    float bmin[3]; float pad0;
+    float bmax[3]; float pad1;
+    int32_t leftChild, rightChild, primStart, primCount;
+} rc_bvh_node;
+
+typedef struct rc_material {         /* 64 B, Material.java:61-66              */
+    float albedo[3]; float metallic;
+    float emission[3]; float emissionStrength;
+    float roughness, ior, transmission; int32_t albedoTex;
+    float pad[4];
+} rc_material;
+
+typedef struct rc_light {            /* 80 B, SceneUploader.java:173-205       */
+    float v0[3]; float area;
+    float v1[3]; float pad1;
+    float v2[3]; float pad2;
+    float emission[3]; float power;  /* emission already scaled by strength    */
+    float normal[3]; float cdf;      /* cdf: cumulative power fraction, last = 1 */
+} rc_light;
+
+/* Camera + lens uniforms of one pass (ThinLensCamera.uploadTo,
+ * src/main/java/com/retrocam/camera/ThinLensCamera.java:27-58, plus the render
+ * resolution that Renderer.java:100-101 sets). */
+typedef struct rc_camera {
+    float camPos[3];
+    float camRight[3];
+    float camUp[3];
+    float camForward[3];
+    float focalDist;        /* TemporalState.focalDist (metres)                   */
+    float lensRadius;       /* focalLength / (2 N), 0 = pinhole                   */
+    float fovTanHalf;       /* ThinLensCamera.tanHalfFov                          */
+    int32_t apertureBlades; /* N-gon bokeh blades (>= 3 when lensRadius > 0)      */
+    float saStrength;       /* spherical aberration                               */
+    float lcaDelta[3];      /* per-channel focal offsets (R, G, B)                */
+    int32_t width, height;  /* render resolution in pixels                        */
+} rc_camera;
+
+/* Path-trace switches (Renderer.java:93-96). */
+typedef struct rc_pt_params {
+    int32_t neeEnabled;     /* 1 = next-event estimation at diffuse hits          */
+    float neeFireflyClamp;  /* per-channel NEE clamp, multiplier of light luma    */
+    int32_t maxBounces;     /* 6 in the reference (pathtrace.comp:465); 1..16     */
+} rc_pt_params;
+
+/* Readable buffers for rc_read. */
+enum {
+    RC_BUF_ACCUM = 0,       /* float4 per px: sum of radiance, sample count (RGBA32F) */
+    RC_BUF_GBUFFER = 1,     /* half4 per px: shading normal, hit t (RGBA16F)       */
+    RC_BUF_GALBEDO = 2,     /* half4 per px: first-hit albedo, 1 (RGBA16F)         */
+    RC_BUF_VARIANCE = 3,    /* float per px: sum of luma^2 (R32F)                  */
+    RC_BUF_PRIMARY_ID = 4,  /* int32 per px: triangle index of the camera ray's first
+                               hit in the most recent pass (-1 = miss); triangles are
+                               numbered in upload (BVH leaf) order                 */
+    RC_BUF_GBUFFER_ID = 5,  /* int32 per px: triangle index hit by the G-buffer ray */
+    RC_BUF_ACCUM64 = 6      /* double4 per px: FP64 accumulation (RC_ACCUM_FP64)   */
+};
+
+/* Accumulation precision (rc_set_accum_mode). */
+enum {
+    RC_ACCUM_FP32 = 0,      /* reference behaviour: RGBA32F running sum            */
+    RC_ACCUM_FP64 = 1       /* FP64 running sum; RC_BUF_ACCUM returns it rounded   */
+};
+
+typedef struct rc_ctx rc_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on HIP device `device`. Replaces the GL objects the
+ * Renderer constructor allocates (Renderer.java:55-62). */
+int rc_create(int device, rc_ctx **out);
+int rc_destroy(rc_ctx *ctx);
+/* Message of the last failure on ctx (or of the last rc_create when ctx is NULL). */
+const char *rc_last_error(const rc_ctx *ctx);
+/* The HIP stream (hipStream_t) all work of ctx is issued on. */
+void *rc_stream(rc_ctx *ctx);
+/* Waits for all work issued on ctx. */
+int rc_sync(rc_ctx *ctx);
+
+/* ---- scene -------------------------------------------------------------- */
+/* SceneUploader.upload/bind (SceneUploader.java:54-115). Streams in the wire
+ * layouts above; nLights may be 0 (the reference uploads a zero stub,
+ * SceneUploader.java:188-196, and every light path early-outs). Indices are
+ * validated: child/prim/material indices out of range return RC_ERR_INVALID_ARG. */
+int rc_upload_scene(rc_ctx *ctx,
+                    const float *tris, int32_t nTris,
+                    const float *nodes, int32_t nNodes,
+                    const float *mats, int32_t nMats,
+                    const float *lights, int32_t nLights,
+                    float totalLightPower);
+
+/* ---- progressive path tracing ------------------------------------------- */
+/* Renderer.render (Renderer.java:72-116): one sample per pixel with
+ * frameIndex = total samples so far, then total samples += 1. A change of
+ * camera->width/height reallocates the images and resets the count. */
+int rc_render_pass(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *params);
+/* nSamples consecutive rc_render_pass calls fused into one launch; results are
+ * identical to the loop. */
+int rc_render_passes(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *params,
+                     int32_t nSamples);
+/* Renderer.reset (Renderer.java:120-130): drop the accumulation, count = 0. */
+int rc_reset(rc_ctx *ctx);
+/* Renderer.getTotalSamples (Renderer.java:138). */
+int32_t rc_total_samples(const rc_ctx *ctx);
+/* RC_ACCUM_FP32 (default) or RC_ACCUM_FP64; resets the accumulation. */
+int rc_set_accum_mode(rc_ctx *ctx, int32_t mode);
+/* Copies a device buffer (RC_BUF_*) of the current resolution to host memory;
+ * `bytes` must equal the buffer size. Replaces glGetTexImage. */
+int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes);
+/* Size in bytes of a readable buffer at the current resolution. */
+size_t rc_buffer_bytes(const rc_ctx *ctx, int32_t which);
+
+/* ---- instrumentation (no counterpart in the reference) ------------------ */
+/* Work counters of the path-trace kernel, for the roofline byte model. Copies
+ * the first n uint64 counters gathered since the previous call into out (order:
+ * rays, shadowRays, nodePops, childTests, triTests, hitUpdates, shadeHits,
+ * shadowPops, shadowTriTests, lightScans, lightReads, paths), clears them, and
+ * switches counting on (enable = 1) or off (enable = 0) for later launches.
+ * Counting uses a separately compiled kernel variant; results are unchanged. */
+int rc_debug_counters(rc_ctx *ctx, int32_t enable, uint64_t *out, int32_t n);
+/* Device PCG streams: for each (x, y, frame) triple in xyf[3*i..], writes the
+ * seed initSeed(x, y, frame) (pathtrace.comp:94-98) followed by `count` pcg()
+ * outputs (pathtrace.comp:80-84) to out[i*(count+1) ..]. n <= 2^20, count <= 1024. */
+int rc_debug_pcg(rc_ctx *ctx, const int32_t *xyf, int32_t n, int32_t count, uint32_t *out);
+/* Device BVH queries on the uploaded scene. rays: n records of 8 floats
+ * {origin.xyz, maxDist, dir.xyz, unused}. Writes the closest-hit triangle
+ * (traverseBVH, pathtrace.comp:142-203; -1 = miss) and its t (1e30 on a miss),
+ * and the shadow-ray result (shadowRay, pathtrace.comp:211-242; 1 = unoccluded
+ * within maxDist). n <= 2^22. */
+int rc_debug_rays(rc_ctx *ctx, const float *rays, int32_t n, int32_t *triOut, float *tOut,
+                  int32_t *unoccludedOut);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RETROCAM_HIP_H */

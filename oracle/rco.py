@@ -1,0 +1,171 @@
+"""ctypes loader of the CPU oracle (oracle/_build/librc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librc_oracle.so")
+
+_FP = C.POINTER(C.c_float)
+_VP = C.c_void_p
+
+
+class RcoScene(C.Structure):
+    _fields_ = [("tris", _FP), ("nodes", _FP), ("mats", _FP), ("lights", _FP),
+                ("nTris", C.c_int32), ("nNodes", C.c_int32), ("nMats", C.c_int32), ("nLights", C.c_int32),
+                ("totalLightPower", C.c_float)]
+
+
+class RcoFrame(C.Structure):
+    _fields_ = [("accum", _VP), ("variance", _VP), ("gbuf", _VP), ("galbedo", _VP),
+                ("primId", _VP), ("gbufId", _VP), ("accum64", _VP)]
+
+
+STAT_FIELDS = ["paths", "rays", "shadowRays", "nodePops", "childTests", "triTests", "hitUpdates", "shadeHits",
+               "shadowPops", "shadowTriTests", "lightScans", "lightReads"]
+
+
+class RcoStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in STAT_FIELDS]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n in STAT_FIELDS}
+
+
+def build() -> str:
+    r = subprocess.run(["make", "-s", "-C", HERE], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from retrocam_amd._native import RcCamera, RcPtParams
+        L = C.CDLL(LIB_PATH)
+        L.rco_pcg.restype = C.c_uint32
+        L.rco_pcg.argtypes = [C.POINTER(C.c_uint32)]
+        L.rco_init_seed.restype = C.c_uint32
+        L.rco_init_seed.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+        L.rco_rand.restype = C.c_float
+        L.rco_rand.argtypes = [C.POINTER(C.c_uint32)]
+        L.rco_closest_hit.restype = C.c_int32
+        L.rco_closest_hit.argtypes = [C.POINTER(RcoScene), _FP, _FP, _FP, _FP]
+        L.rco_brute_closest.restype = C.c_int32
+        L.rco_brute_closest.argtypes = [C.POINTER(RcoScene), _FP, _FP, _FP]
+        L.rco_shadow_ray.restype = C.c_int32
+        L.rco_shadow_ray.argtypes = [C.POINTER(RcoScene), _FP, _FP, C.c_float]
+        L.rco_render_pass.restype = C.c_int32
+        L.rco_render_pass.argtypes = [C.POINTER(RcoScene), C.POINTER(RcCamera), C.POINTER(RcPtParams),
+                                      C.c_int32, C.c_int32, C.POINTER(RcoFrame), C.POINTER(RcoStats)]
+        L.rco_fmath.restype = C.c_int32
+        L.rco_fmath.argtypes = [C.c_int32, _FP, _FP, _FP, C.c_int32]
+        L.rco_num_threads.restype = C.c_int32
+        L.rco_num_threads.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(_VP)
+
+
+class OracleScene:
+    """Holds the float streams alive and the rco_scene view of them."""
+
+    def __init__(self, sc):
+        self.tris = np.ascontiguousarray(sc.tris, dtype=np.float32)
+        self.nodes = np.ascontiguousarray(sc.nodes, dtype=np.float32)
+        self.mats = np.ascontiguousarray(sc.mats, dtype=np.float32)
+        lights = sc.lights if sc.lights.shape[0] > 0 else np.zeros((1, 20), np.float32)
+        self.lights = np.ascontiguousarray(lights, dtype=np.float32)
+        self.c = RcoScene(_fp(self.tris), _fp(self.nodes), _fp(self.mats), _fp(self.lights),
+                          self.tris.shape[0], self.nodes.shape[0], self.mats.shape[0], sc.lights.shape[0],
+                          float(sc.totalLightPower))
+
+
+class OracleRenderer:
+    """Progressive accumulation on the CPU with the semantics of Renderer.render."""
+
+    def __init__(self, scene, cam_c, params_c, stride: int = 1, fp64: bool = False):
+        self.scene = OracleScene(scene)
+        self.cam = cam_c
+        self.params = params_c
+        self.stride = stride
+        W, H = cam_c.width, cam_c.height
+        n = W * H
+        self.accum = np.zeros((n, 4), np.float32)
+        self.variance = np.zeros(n, np.float32)
+        self.gbuf = np.zeros((n, 4), np.uint16)
+        self.galbedo = np.zeros((n, 4), np.uint16)
+        self.primId = np.full(n, -7, np.int32)
+        self.gbufId = np.full(n, -7, np.int32)
+        self.accum64 = np.zeros((n, 4), np.float64) if fp64 else None
+        self.frame = 0
+        self.stats = RcoStats()
+
+    def render_pass(self, collect_stats: bool = False):
+        f = RcoFrame(_vp(self.accum), _vp(self.variance), _vp(self.gbuf), _vp(self.galbedo), _vp(self.primId),
+                     _vp(self.gbufId), _vp(self.accum64))
+        st = C.byref(self.stats) if collect_stats else None
+        rc = lib().rco_render_pass(C.byref(self.scene.c), C.byref(self.cam), C.byref(self.params), self.frame,
+                                   self.stride, C.byref(f), st)
+        if rc != 0:
+            raise RuntimeError("rco_render_pass failed")
+        self.frame += 1
+
+    def mask(self):
+        W, H = self.cam.width, self.cam.height
+        m = np.zeros((H, W), dtype=bool)
+        m[::self.stride, ::self.stride] = True
+        return m.reshape(-1)
+
+
+def fmath(fn: int, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, np.float32)
+    out = np.zeros_like(x)
+    if lib().rco_fmath(int(fn), _fp(x), _fp(y), _fp(out), x.size) != 0:
+        raise ValueError("bad fmath function id")
+    return out
+
+
+def closest_hit(scene: OracleScene, ro, rd):
+    ro = np.asarray(ro, np.float32)
+    rd = np.asarray(rd, np.float32)
+    t = np.zeros(1, np.float32)
+    n = np.zeros(3, np.float32)
+    tri = lib().rco_closest_hit(C.byref(scene.c), _fp(ro), _fp(rd), _fp(t), _fp(n))
+    return tri, float(t[0]), n
+
+
+def brute_closest(scene: OracleScene, ro, rd):
+    ro = np.asarray(ro, np.float32)
+    rd = np.asarray(rd, np.float32)
+    t = np.zeros(1, np.float32)
+    tri = lib().rco_brute_closest(C.byref(scene.c), _fp(ro), _fp(rd), _fp(t))
+    return tri, float(t[0])
+
+
+def shadow_ray(scene: OracleScene, ro, rd, max_dist):
+    ro = np.asarray(ro, np.float32)
+    rd = np.asarray(rd, np.float32)
+    return bool(lib().rco_shadow_ray(C.byref(scene.c), _fp(ro), _fp(rd), float(max_dist)))

@@ -1,0 +1,141 @@
+"""ctypes bindings of libretrocam_hip.so (C ABI in include/retrocam_hip.h) and
+librc_host.so (include/rc_host.h).
+
+The HIP library is loaded from retrocam_amd/lib/ only; a missing library raises
+at import of the binding (there is no CPU fallback for the render path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+HIP_LIB_PATH = os.path.join(LIBDIR, "libretrocam_hip.so")
+HOST_LIB_PATH = os.path.join(LIBDIR, "librc_host.so")
+
+# status codes
+RC_OK = 0
+RC_ERR_INVALID_ARG = 1
+RC_ERR_NO_DEVICE = 2
+RC_ERR_HIP = 3
+RC_ERR_OOM = 4
+RC_ERR_STATE = 5
+RC_ERR_LIMIT = 6
+
+# buffers
+RC_BUF_ACCUM = 0
+RC_BUF_GBUFFER = 1
+RC_BUF_GALBEDO = 2
+RC_BUF_VARIANCE = 3
+RC_BUF_PRIMARY_ID = 4
+RC_BUF_GBUFFER_ID = 5
+RC_BUF_ACCUM64 = 6
+
+RC_ACCUM_FP32 = 0
+RC_ACCUM_FP64 = 1
+
+F3 = C.c_float * 3
+
+
+class RcCamera(C.Structure):
+    _fields_ = [
+        ("camPos", F3), ("camRight", F3), ("camUp", F3), ("camForward", F3),
+        ("focalDist", C.c_float), ("lensRadius", C.c_float), ("fovTanHalf", C.c_float),
+        ("apertureBlades", C.c_int32), ("saStrength", C.c_float), ("lcaDelta", F3),
+        ("width", C.c_int32), ("height", C.c_int32),
+    ]
+
+
+class RcPtParams(C.Structure):
+    _fields_ = [("neeEnabled", C.c_int32), ("neeFireflyClamp", C.c_float), ("maxBounces", C.c_int32)]
+
+
+class RcsInfo(C.Structure):
+    _fields_ = [
+        ("nTris", C.c_int32), ("nNodes", C.c_int32), ("nMats", C.c_int32), ("nLights", C.c_int32),
+        ("totalLightPower", C.c_float), ("maxDepth", C.c_int32), ("maxLeafSize", C.c_int32),
+        ("nLeaves", C.c_int32),
+    ]
+
+
+_FP = C.POINTER(C.c_float)
+_VP = C.c_void_p
+
+# name -> (restype, argtypes); the full exported surface of include/retrocam_hip.h
+HIP_SYMBOLS = {
+    "rc_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rc_destroy": (C.c_int, [_VP]),
+    "rc_last_error": (C.c_char_p, [_VP]),
+    "rc_stream": (C.c_void_p, [_VP]),
+    "rc_sync": (C.c_int, [_VP]),
+    "rc_upload_scene": (C.c_int, [_VP, _FP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32,
+                                  C.c_float]),
+    "rc_render_pass": (C.c_int, [_VP, C.POINTER(RcCamera), C.POINTER(RcPtParams)]),
+    "rc_render_passes": (C.c_int, [_VP, C.POINTER(RcCamera), C.POINTER(RcPtParams), C.c_int32]),
+    "rc_reset": (C.c_int, [_VP]),
+    "rc_total_samples": (C.c_int32, [_VP]),
+    "rc_set_accum_mode": (C.c_int, [_VP, C.c_int32]),
+    "rc_read": (C.c_int, [_VP, C.c_int32, _VP, C.c_size_t]),
+    "rc_buffer_bytes": (C.c_size_t, [_VP, C.c_int32]),
+    "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),
+    "rc_debug_pcg": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
+    "rc_debug_rays": (C.c_int, [_VP, _FP, C.c_int32, C.POINTER(C.c_int32), _FP, C.POINTER(C.c_int32)]),
+}
+
+HOST_SYMBOLS = {
+    "rcs_new": (C.c_void_p, []),
+    "rcs_free": (None, [_VP]),
+    "rcs_add_material": (C.c_int32, [_VP, _FP]),
+    "rcs_add_box": (C.c_int32, [_VP] + [C.c_float] * 6 + [C.c_int32]),
+    "rcs_add_sphere": (C.c_int32, [_VP] + [C.c_float] * 4 + [C.c_int32] * 3),
+    "rcs_add_quad": (C.c_int32, [_VP, _FP, C.c_int32]),
+    "rcs_rotate_since": (C.c_int32, [_VP, C.c_int32] + [C.c_float] * 6),
+    "rcs_add_triangles": (C.c_int32, [_VP, _FP, C.c_int32]),
+    "rcs_triangle_count": (C.c_int32, [_VP]),
+    "rcs_build": (C.c_int32, [_VP]),
+    "rcs_get_info": (C.c_int32, [_VP, C.POINTER(RcsInfo)]),
+    "rcs_copy_tris": (C.c_int32, [_VP, _FP]),
+    "rcs_copy_nodes": (C.c_int32, [_VP, _FP]),
+    "rcs_copy_mats": (C.c_int32, [_VP, _FP]),
+    "rcs_copy_lights": (C.c_int32, [_VP, _FP]),
+    "rcs_orbit_basis": (None, [C.c_float] * 6 + [_FP] * 4),
+    "rcs_lookat_basis": (None, [_FP] * 5),
+    "rcs_tan_half_fov": (C.c_float, [C.c_float]),
+}
+
+
+def _bind(lib: C.CDLL, table: dict) -> C.CDLL:
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_hip = None
+_host = None
+
+
+def hip_lib() -> C.CDLL:
+    """The loaded libretrocam_hip.so; raises if it has not been built."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB_PATH):
+            raise RuntimeError(
+                f"{HIP_LIB_PATH} is missing: build it with `python -m retrocam_amd.build` "
+                "(the render path has no CPU fallback)")
+        _hip = _bind(C.CDLL(HIP_LIB_PATH), HIP_SYMBOLS)
+    return _hip
+
+
+def host_lib() -> C.CDLL:
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise RuntimeError(f"{HOST_LIB_PATH} is missing: build it with `python -m retrocam_amd.build`")
+        _host = _bind(C.CDLL(HOST_LIB_PATH), HOST_SYMBOLS)
+    return _host
+
+
+def fptr(a) -> "C.POINTER(C.c_float)":
+    return a.ctypes.data_as(_FP)

@@ -1,0 +1,420 @@
+// rc_capi.hip — the extern "C" entry points of libretrocam_hip.so: context,
+// scene upload (wire layout -> device layout), progressive passes and readback.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rc_internal.h"
+#include "retrocam_hip.h"
+
+namespace rc {
+
+static thread_local std::string g_create_error;
+
+int set_error(rc_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(rc_ctx *ctx, hipError_t e, const char *what) {
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s failed: %s (%d)", what, hipGetErrorString(e), (int)e);
+    return set_error(ctx, e == hipErrorOutOfMemory ? RC_ERR_OOM : RC_ERR_HIP, buf);
+}
+
+template <typename T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+static void free_scene(DeviceScene &d) {
+    dfree(d.inner); dfree(d.triPos); dfree(d.triShade); dfree(d.mats);
+    dfree(d.lights); dfree(d.cdf); dfree(d.leafBigCount);
+    d = DeviceScene{};
+}
+
+static void free_images(Images &im) {
+    dfree(im.accum); dfree(im.accum64); dfree(im.gbuf); dfree(im.galbedo);
+    dfree(im.variance); dfree(im.primId); dfree(im.gbufId);
+    im = Images{};
+}
+
+template <typename T>
+static int dalloc(rc_ctx *ctx, T *&p, size_t count, const char *what) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+    if (e != hipSuccess) return hip_fail(ctx, e, what);
+    p = (T *)q;
+    return RC_OK;
+}
+
+static int upload(rc_ctx *ctx, void *dst, const void *src, size_t bytes, const char *what) {
+    if (bytes == 0) return RC_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, what);
+    return RC_OK;
+}
+
+static int ensure_images(rc_ctx *ctx, int w, int h) {
+    Images &im = ctx->img;
+    bool need64 = ctx->accumMode == RC_ACCUM_FP64;
+    if (im.width == w && im.height == h && (!need64 || im.accum64)) return RC_OK;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    free_images(im);
+    size_t n = (size_t)w * (size_t)h;
+    int rc;
+    if ((rc = dalloc(ctx, im.accum, n, "hipMalloc(accum)"))) return rc;
+    if ((rc = dalloc(ctx, im.gbuf, n, "hipMalloc(gbuffer)"))) return rc;
+    if ((rc = dalloc(ctx, im.galbedo, n, "hipMalloc(galbedo)"))) return rc;
+    if ((rc = dalloc(ctx, im.variance, n, "hipMalloc(variance)"))) return rc;
+    if ((rc = dalloc(ctx, im.primId, n, "hipMalloc(primId)"))) return rc;
+    if ((rc = dalloc(ctx, im.gbufId, n, "hipMalloc(gbufId)"))) return rc;
+    if (need64 && (rc = dalloc(ctx, im.accum64, n, "hipMalloc(accum64)"))) return rc;
+    // Zero-initialised like a fresh GL texture allocation with NULL data.
+    size_t sizes[] = {n * 16, n * 8, n * 8, n * 4, n * 4, n * 4};
+    void *ptrs[] = {im.accum, im.gbuf, im.galbedo, im.variance, im.primId, im.gbufId};
+    for (int i = 0; i < 6; ++i) {
+        e = hipMemsetAsync(ptrs[i], 0, sizes[i], ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(images)");
+    }
+    if (im.accum64) {
+        e = hipMemsetAsync(im.accum64, 0, n * 32, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(accum64)");
+    }
+    im.width = w;
+    im.height = h;
+    ctx->totalSamples = 0;
+    return RC_OK;
+}
+
+static inline int32_t bits_i(float f) { int32_t i; std::memcpy(&i, &f, 4); return i; }
+static inline float bits_f(int32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+
+static int validate_camera(rc_ctx *ctx, const rc_camera *cam) {
+    if (!cam) return set_error(ctx, RC_ERR_INVALID_ARG, "camera is NULL");
+    if (cam->width <= 0 || cam->height <= 0 || cam->width > 16384 || cam->height > 16384)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "camera width/height must be in 1..16384");
+    if (cam->lensRadius > 0.0f && cam->apertureBlades < 1)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "apertureBlades must be >= 1 when lensRadius > 0");
+    return RC_OK;
+}
+
+}  // namespace rc
+
+using namespace rc;
+
+extern "C" {
+
+int rc_create(int device, rc_ctx **out) {
+    if (!out) return set_error(nullptr, RC_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return set_error(nullptr, RC_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return set_error(nullptr, RC_ERR_INVALID_ARG, "device index out of range");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return set_error(nullptr, RC_ERR_HIP, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        std::string m = "device is ";
+        m += prop.gcnArchName;
+        m += "; libretrocam_hip.so is built for gfx950 only";
+        return set_error(nullptr, RC_ERR_NO_DEVICE, m);
+    }
+    rc_ctx *ctx = new (std::nothrow) rc_ctx();
+    if (!ctx) return set_error(nullptr, RC_ERR_OOM, "host allocation failed");
+    ctx->device = device;
+    e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return set_error(nullptr, RC_ERR_HIP, "hipStreamCreate failed");
+    }
+    void *c = nullptr;
+    e = hipMalloc(&c, 64 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_error(nullptr, RC_ERR_OOM, "hipMalloc(counters) failed");
+    }
+    ctx->counters = (unsigned long long *)c;
+    (void)hipMemsetAsync(ctx->counters, 0, 64 * sizeof(unsigned long long), ctx->stream);
+    *out = ctx;
+    return RC_OK;
+}
+
+int rc_destroy(rc_ctx *ctx) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_scene(ctx->scene);
+    free_images(ctx->img);
+    dfree(ctx->counters);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RC_OK;
+}
+
+const char *rc_last_error(const rc_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+void *rc_stream(rc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int rc_sync(rc_ctx *ctx) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    return RC_OK;
+}
+
+int rc_upload_scene(rc_ctx *ctx, const float *tris, int32_t nTris, const float *nodes, int32_t nNodes,
+                    const float *mats, int32_t nMats, const float *lights, int32_t nLights,
+                    float totalLightPower) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (nTris <= 0 || nNodes <= 0 || nMats <= 0 || nLights < 0)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "nTris, nNodes and nMats must be > 0, nLights >= 0");
+    if (!tris || !nodes || !mats || (nLights > 0 && !lights))
+        return set_error(ctx, RC_ERR_INVALID_ARG, "NULL scene stream");
+    if (nTris >= (1 << 26))
+        return set_error(ctx, RC_ERR_LIMIT, "nTris must be < 2^26");
+    (void)hipSetDevice(ctx->device);
+
+    // --- validate the BVH: children in range, strictly after the parent, each
+    //     node reached once (a tree), leaves in range.
+    std::vector<int> parents(nNodes, 0);
+    std::vector<int32_t> L(nNodes), R(nNodes), PS(nNodes), PC(nNodes);
+    for (int i = 0; i < nNodes; ++i) {
+        const float *n = nodes + 12 * (size_t)i;
+        L[i] = bits_i(n[8]); R[i] = bits_i(n[9]); PS[i] = bits_i(n[10]); PC[i] = bits_i(n[11]);
+        if (PC[i] > 0) {
+            if (PS[i] < 0 || (int64_t)PS[i] + PC[i] > nTris)
+                return set_error(ctx, RC_ERR_INVALID_ARG, "BVH leaf primitive range out of bounds");
+        } else {
+            if (L[i] <= i || R[i] <= i || L[i] >= nNodes || R[i] >= nNodes || L[i] == R[i])
+                return set_error(ctx, RC_ERR_INVALID_ARG, "BVH child index invalid (must be > parent, < nNodes)");
+            parents[L[i]]++;
+            parents[R[i]]++;
+        }
+    }
+    for (int i = 1; i < nNodes; ++i)
+        if (parents[i] > 1) return set_error(ctx, RC_ERR_INVALID_ARG, "BVH node has more than one parent");
+    for (int i = 0; i < nTris; ++i) {
+        int32_t m = bits_i(tris[32 * (size_t)i + 30]);
+        if (m < 0 || m >= nMats) return set_error(ctx, RC_ERR_INVALID_ARG, "triangle matIndex out of range");
+    }
+
+    // --- inner-node numbering (DFS order preserved) and child refs
+    std::vector<int> innerIdx(nNodes, -1);
+    int nInner = 0, nLeaves = 0;
+    for (int i = 0; i < nNodes; ++i) {
+        if (PC[i] > 0) ++nLeaves;
+        else innerIdx[i] = nInner++;
+    }
+    std::vector<int> leafBig(nTris, 0);
+    bool anyBig = false;
+    auto refOf = [&](int node) -> int32_t {
+        if (PC[node] > 0) {
+            uint32_t cnt = PC[node] <= 31 ? (uint32_t)PC[node] : 0u;
+            if (cnt == 0) { leafBig[PS[node]] = PC[node]; anyBig = true; }
+            return (int32_t)~(((uint32_t)PS[node] << 5) | cnt);
+        }
+        return innerIdx[node];
+    };
+    std::vector<float4> inner((size_t)nInner * 4);
+    for (int i = 0; i < nNodes; ++i) {
+        if (PC[i] > 0) continue;
+        const float *l = nodes + 12 * (size_t)L[i];
+        const float *r = nodes + 12 * (size_t)R[i];
+        float4 *o = &inner[(size_t)innerIdx[i] * 4];
+        o[0] = make_float4(l[0], l[1], l[2], l[4]);
+        o[1] = make_float4(l[5], l[6], r[0], r[1]);
+        o[2] = make_float4(r[2], r[4], r[5], r[6]);
+        o[3] = make_float4(bits_f(refOf(L[i])), bits_f(refOf(R[i])), 0.0f, 0.0f);
+    }
+    int32_t rootRef = refOf(0);
+    // depth
+    int maxDepth = 0;
+    {
+        std::vector<std::pair<int, int>> st;
+        st.push_back({0, 0});
+        while (!st.empty()) {
+            auto pr = st.back();
+            st.pop_back();
+            if (pr.second > maxDepth) maxDepth = pr.second;
+            if (PC[pr.first] <= 0) {
+                st.push_back({L[pr.first], pr.second + 1});
+                st.push_back({R[pr.first], pr.second + 1});
+            }
+        }
+    }
+
+    std::vector<float4> pos((size_t)nTris * 3), shade((size_t)nTris * 3);
+    for (int i = 0; i < nTris; ++i) {
+        const float *t = tris + 32 * (size_t)i;
+        pos[3 * i + 0] = make_float4(t[0], t[1], t[2], 0.0f);
+        pos[3 * i + 1] = make_float4(t[4] - t[0], t[5] - t[1], t[6] - t[2], 0.0f);
+        pos[3 * i + 2] = make_float4(t[8] - t[0], t[9] - t[1], t[10] - t[2], 0.0f);
+        shade[3 * i + 0] = make_float4(t[12], t[13], t[14], t[30]);
+        shade[3 * i + 1] = make_float4(t[16], t[17], t[18], 0.0f);
+        shade[3 * i + 2] = make_float4(t[20], t[21], t[22], 0.0f);
+    }
+    std::vector<float> cdf((size_t)(nLights > 0 ? nLights : 1), 1.0f);
+    for (int i = 0; i < nLights; ++i) cdf[i] = lights[20 * (size_t)i + 19];
+
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    free_scene(ctx->scene);
+    DeviceScene &D = ctx->scene;
+    int rc;
+    if ((rc = dalloc(ctx, D.inner, inner.size() ? inner.size() : 4, "hipMalloc(nodes)"))) return rc;
+    if ((rc = dalloc(ctx, D.triPos, pos.size(), "hipMalloc(triangles)"))) return rc;
+    if ((rc = dalloc(ctx, D.triShade, shade.size(), "hipMalloc(shading)"))) return rc;
+    if ((rc = dalloc(ctx, D.mats, (size_t)nMats * 4, "hipMalloc(materials)"))) return rc;
+    if ((rc = dalloc(ctx, D.lights, (size_t)(nLights > 0 ? nLights : 1) * 5, "hipMalloc(lights)"))) return rc;
+    if ((rc = dalloc(ctx, D.cdf, cdf.size(), "hipMalloc(cdf)"))) return rc;
+    if ((rc = dalloc(ctx, D.leafBigCount, (size_t)nTris, "hipMalloc(leafBigCount)"))) return rc;
+    if ((rc = upload(ctx, D.inner, inner.data(), inner.size() * sizeof(float4), "upload(nodes)"))) return rc;
+    if ((rc = upload(ctx, D.triPos, pos.data(), pos.size() * sizeof(float4), "upload(triangles)"))) return rc;
+    if ((rc = upload(ctx, D.triShade, shade.data(), shade.size() * sizeof(float4), "upload(shading)"))) return rc;
+    if ((rc = upload(ctx, D.mats, mats, (size_t)nMats * 64, "upload(materials)"))) return rc;
+    if (nLights > 0 && (rc = upload(ctx, D.lights, lights, (size_t)nLights * 80, "upload(lights)"))) return rc;
+    if ((rc = upload(ctx, D.cdf, cdf.data(), cdf.size() * sizeof(float), "upload(cdf)"))) return rc;
+    if ((rc = upload(ctx, D.leafBigCount, leafBig.data(), leafBig.size() * sizeof(int), "upload(leaves)"))) return rc;
+    (void)anyBig;
+    e = hipStreamSynchronize(ctx->stream);  // host vectors go out of scope
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    D.nTris = nTris; D.nInner = nInner; D.nLeaves = nLeaves; D.nMats = nMats; D.nLights = nLights;
+    D.totalLightPower = totalLightPower;
+    for (int k = 0; k < 3; ++k) { D.rootMin[k] = nodes[k]; D.rootMax[k] = nodes[4 + k]; }
+    D.rootRef = rootRef;
+    D.maxDepth = maxDepth;
+    D.valid = true;
+    return RC_OK;
+}
+
+static int render_common(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *p, int32_t n) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = validate_camera(ctx, cam))) return rc;
+    if (!p) return set_error(ctx, RC_ERR_INVALID_ARG, "params is NULL");
+    if (p->maxBounces < 1 || p->maxBounces > 16)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "maxBounces must be in 1..16");
+    if (n < 1) return set_error(ctx, RC_ERR_INVALID_ARG, "nSamples must be >= 1");
+    if (!ctx->scene.valid) return set_error(ctx, RC_ERR_STATE, "no scene uploaded");
+    if ((int64_t)ctx->totalSamples + n > 0x7fffffff / 8)
+        return set_error(ctx, RC_ERR_LIMIT, "sample count overflow");
+    (void)hipSetDevice(ctx->device);
+    if ((rc = ensure_images(ctx, cam->width, cam->height))) return rc;
+    hipError_t e = launch_pathtrace(ctx, *cam, *p, n);
+    if (e != hipSuccess) return hip_fail(ctx, e, "pathtrace launch");
+    ctx->totalSamples += n;
+    return RC_OK;
+}
+
+int rc_render_pass(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *p) {
+    return render_common(ctx, cam, p, 1);
+}
+
+int rc_render_passes(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *p, int32_t n) {
+    return render_common(ctx, cam, p, n);
+}
+
+int rc_reset(rc_ctx *ctx) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    ctx->totalSamples = 0;
+    // Renderer.reset re-creates the textures with NULL data (zeroed storage).
+    Images &im = ctx->img;
+    if (im.width > 0) {
+        (void)hipSetDevice(ctx->device);
+        size_t n = (size_t)im.width * (size_t)im.height;
+        size_t sizes[] = {n * 16, n * 8, n * 8, n * 4, n * 4, n * 4, n * 32};
+        void *ptrs[] = {im.accum, im.gbuf, im.galbedo, im.variance, im.primId, im.gbufId, im.accum64};
+        for (int i = 0; i < 7; ++i) {
+            if (!ptrs[i]) continue;
+            hipError_t e = hipMemsetAsync(ptrs[i], 0, sizes[i], ctx->stream);
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(reset)");
+        }
+    }
+    return RC_OK;
+}
+
+int rc_debug_counters(rc_ctx *ctx, int32_t enable, uint64_t *out, int32_t n) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (n < 0 || n > 64 || (n > 0 && !out)) return set_error(ctx, RC_ERR_INVALID_ARG, "bad counter buffer");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipSuccess;
+    if (n > 0) {
+        e = hipMemcpyAsync(out, ctx->counters, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "rc_debug_counters(read)");
+    }
+    e = hipMemsetAsync(ctx->counters, 0, 64 * sizeof(unsigned long long), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rc_debug_counters(clear)");
+    ctx->countersEnabled = enable ? 1 : 0;
+    return RC_OK;
+}
+
+int32_t rc_total_samples(const rc_ctx *ctx) { return ctx ? ctx->totalSamples : -1; }
+
+int rc_set_accum_mode(rc_ctx *ctx, int32_t mode) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (mode != RC_ACCUM_FP32 && mode != RC_ACCUM_FP64)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "accumulation mode must be RC_ACCUM_FP32 or RC_ACCUM_FP64");
+    ctx->accumMode = mode;
+    ctx->totalSamples = 0;
+    if (ctx->img.width > 0) {
+        int w = ctx->img.width, h = ctx->img.height;
+        if (mode == RC_ACCUM_FP64 && !ctx->img.accum64) {
+            (void)hipSetDevice(ctx->device);
+            ctx->img.width = 0;  // force reallocation with the FP64 buffer
+            return ensure_images(ctx, w, h);
+        }
+    }
+    return RC_OK;
+}
+
+size_t rc_buffer_bytes(const rc_ctx *ctx, int32_t which) {
+    if (!ctx) return 0;
+    size_t n = (size_t)ctx->img.width * (size_t)ctx->img.height;
+    switch (which) {
+        case RC_BUF_ACCUM: return n * 16;
+        case RC_BUF_GBUFFER: return n * 8;
+        case RC_BUF_GALBEDO: return n * 8;
+        case RC_BUF_VARIANCE: return n * 4;
+        case RC_BUF_PRIMARY_ID: return n * 4;
+        case RC_BUF_GBUFFER_ID: return n * 4;
+        case RC_BUF_ACCUM64: return ctx->img.accum64 ? n * 32 : 0;
+        default: return 0;
+    }
+}
+
+int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (!dst) return set_error(ctx, RC_ERR_INVALID_ARG, "dst is NULL");
+    size_t need = rc_buffer_bytes(ctx, which);
+    if (need == 0) return set_error(ctx, RC_ERR_STATE, "buffer not available (bad id, no render yet, or FP64 off)");
+    if (bytes != need) return set_error(ctx, RC_ERR_INVALID_ARG, "bytes does not match the buffer size");
+    const void *src = nullptr;
+    switch (which) {
+        case RC_BUF_ACCUM: src = ctx->img.accum; break;
+        case RC_BUF_GBUFFER: src = ctx->img.gbuf; break;
+        case RC_BUF_GALBEDO: src = ctx->img.galbedo; break;
+        case RC_BUF_VARIANCE: src = ctx->img.variance; break;
+        case RC_BUF_PRIMARY_ID: src = ctx->img.primId; break;
+        case RC_BUF_GBUFFER_ID: src = ctx->img.gbufId; break;
+        case RC_BUF_ACCUM64: src = ctx->img.accum64; break;
+    }
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rc_read");
+    return RC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+// rc_device.h — gfx950 device building blocks of the render hot path:
+// vector helpers, PCG, ray/box and ray/triangle tests and the two BVH walks.
+//
+// Device data layout (built by rc_upload_scene from the Java float streams):
+//   inner nodes  64 B  {L.min.xyz, L.max.x} {L.max.yz, R.min.xy} {R.min.z, R.max.xyz}
+//                      {leftRef, rightRef, -, -}
+//                a BVH2 node that holds both children's boxes, so the ordered
+//                child test of pathtrace.comp:181-197 is one 64 B read.
+//   child ref    >= 0: index of an inner node;  < 0: leaf, ~ref = start<<5 | count
+//                (count 0 means "look up leafBigCount[start]", for leaves > 31 tris)
+//   triangles    48 B  {v0, -} {v1 - v0, -} {v2 - v0, -}  (MT test operands)
+//   shading      48 B  {n0, matIndex bits} {n1, -} {n2, -}
+//   materials    64 B  Material.write layout;  lights 80 B + a packed cdf[] array
+//
+// Every arithmetic step follows the reference GLSL operand order with no FMA
+// contraction (-ffp-contract=off) so that results equal the CPU oracle bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rc_fmath.h"
+
+namespace rc {
+
+struct v3 { float x, y, z; };
+
+__device__ __forceinline__ v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 operator*(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 operator/(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ v3 normalize(v3 a) { float inv = 1.0f / sqrtf(dot(a, a)); return a * inv; }
+__device__ __forceinline__ v3 reflect(v3 i, v3 n) { float d = 2.0f * dot(n, i); return i - n * d; }
+__device__ __forceinline__ v3 refract(v3 i, v3 n, float eta) {
+    float d = dot(n, i);
+    float k = 1.0f - (eta * eta) * (1.0f - d * d);
+    if (k < 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    float c = eta * d + sqrtf(k);
+    return i * eta - n * c;
+}
+__device__ __forceinline__ v3 xyz(float4 f) { return mk(f.x, f.y, f.z); }
+
+// PCG-RXS-M-XS (pathtrace.comp:80-98).
+__device__ __forceinline__ uint32_t pcg(uint32_t &s) {
+    s = s * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+__device__ __forceinline__ float rnd(uint32_t &s) { return (float)pcg(s) / 4294967296.0f; }
+__device__ __forceinline__ uint32_t init_seed(int x, int y, int frame) {
+    uint32_t s = (uint32_t)x + (uint32_t)y * 7919u + (uint32_t)frame * 6271u;
+    pcg(s);
+    pcg(s);
+    return s;
+}
+
+struct SceneView {
+    const float4 *__restrict__ inner;    // 4 per inner node
+    const float4 *__restrict__ triPos;   // 3 per triangle
+    const float4 *__restrict__ triShade; // 3 per triangle
+    const float4 *__restrict__ mats;     // 4 per material
+    const float4 *__restrict__ lights;   // 5 per light
+    const float *__restrict__ cdf;       // 1 per light
+    const int *__restrict__ leafBigCount;
+    float rootMin[3], rootMax[3];
+    int rootRef;
+    int nLights;
+    float totalLightPower;
+};
+
+// Work counters (bench instrumentation); summed per wave into global memory.
+struct Counters {
+    uint32_t rays, shadowRays, nodePops, childTests, triTests, hitUpdates, shadeHits,
+        shadowPops, shadowTriTests, lightScans, lightReads, paths;
+};
+
+template <bool STATS>
+struct Ctr {
+    Counters c;
+    __device__ void zero() { c = Counters{}; }
+};
+
+// intersectAABB (pathtrace.comp:102-110).
+__device__ __forceinline__ bool aabb(v3 ro, v3 inv, float mnx, float mny, float mnz,
+                                     float mxx, float mxy, float mxz, float &tmin) {
+    float t0x = (mnx - ro.x) * inv.x, t0y = (mny - ro.y) * inv.y, t0z = (mnz - ro.z) * inv.z;
+    float t1x = (mxx - ro.x) * inv.x, t1y = (mxy - ro.y) * inv.y, t1z = (mxz - ro.z) * inv.z;
+    float lx = fminf(t0x, t1x), ly = fminf(t0y, t1y), lz = fminf(t0z, t1z);
+    float hx = fmaxf(t0x, t1x), hy = fmaxf(t0y, t1y), hz = fmaxf(t0z, t1z);
+    float tn = fmaxf(fmaxf(lx, ly), lz);
+    float tx = fminf(fminf(hx, hy), hz);
+    tmin = tn;
+    return tx >= fmaxf(tn, 0.0f);
+}
+
+// intersectTriangle (pathtrace.comp:112-130) on pre-subtracted edges.
+__device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int idx,
+                                         float &t, float &bu, float &bv) {
+    const float4 *p = S.triPos + 3 * (size_t)idx;
+    float4 a = p[0], b = p[1], c = p[2];
+    v3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
+    v3 h = cross(rd, e2);
+    float det = dot(e1, h);
+    if (fabsf(det) < 1e-8f) return false;
+    float f = 1.0f / det;
+    v3 s = ro - v0;
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    v3 q = cross(s, e1);
+    float v = f * dot(rd, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float tt = f * dot(e2, q);
+    if (tt < 1e-4f) return false;
+    t = tt; bu = u; bv = v;
+    return true;
+}
+
+__device__ __forceinline__ void leaf_range(const SceneView &S, int ref, int &start, int &count) {
+    uint32_t v = ~(uint32_t)ref;
+    start = (int)(v >> 5);
+    count = (int)(v & 31u);
+    if (count == 0) count = S.leafBigCount[start];
+}
+
+struct Hit {
+    float t, bu, bv;
+    int tri;
+};
+
+// traverseBVH (pathtrace.comp:142-203). The stack holds (ref, tEntry): the child
+// box test at push time is the same computation as the reference's re-test of the
+// node's own box at pop, so the pop-time cull `aabbT > rec.t` reads tEntry.
+// The 62-entry cap (pathtrace.comp:176) is applied identically.
+template <bool STATS>
+__device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Counters &C) {
+    Hit h;
+    h.t = 1e30f; h.tri = -1; h.bu = 0.0f; h.bv = 0.0f;
+    v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    if (STATS) { C.rays++; C.nodePops++; }
+    float tr;
+    if (!aabb(ro, inv, S.rootMin[0], S.rootMin[1], S.rootMin[2], S.rootMax[0], S.rootMax[1],
+              S.rootMax[2], tr))
+        return h;
+    int stkRef[64];
+    float stkT[64];
+    int top = 0;
+    int ref = S.rootRef;
+    float tEntry = tr;
+    bool have = true;
+    while (true) {
+        if (!have) {
+            if (top == 0) break;
+            --top;
+            ref = stkRef[top];
+            tEntry = stkT[top];
+            if (STATS) C.nodePops++;
+        }
+        have = false;
+        if (tEntry > h.t) continue;
+        if (ref < 0) {
+            int start, count;
+            leaf_range(S, ref, start, count);
+            for (int i = start; i < start + count; ++i) {
+                float t, bu, bv;
+                if (STATS) C.triTests++;
+                if (tri_test(S, ro, rd, i, t, bu, bv) && t < h.t) {
+                    h.t = t; h.bu = bu; h.bv = bv; h.tri = i;
+                    if (STATS) C.hitUpdates++;
+                }
+            }
+        } else if (top < 62) {
+            const float4 *n = S.inner + 4 * (size_t)ref;
+            float4 a = n[0], b = n[1], c = n[2];
+            int4 r = ((const int4 *)n)[3];
+            if (STATS) C.childTests++;
+            float tl, trr;
+            bool hl = aabb(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
+            bool hr = aabb(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, trr);
+            if (hl && hr) {
+                if (tl < trr) {
+                    stkRef[top] = r.y; stkT[top] = trr; ++top;
+                    ref = r.x; tEntry = tl;
+                } else {
+                    stkRef[top] = r.x; stkT[top] = tl; ++top;
+                    ref = r.y; tEntry = trr;
+                }
+                have = true;  // the entry the reference pushes last is popped next
+                if (STATS) C.nodePops++;
+            } else if (hl) {
+                ref = r.x; tEntry = tl; have = true;
+                if (STATS) C.nodePops++;
+            } else if (hr) {
+                ref = r.y; tEntry = trr; have = true;
+                if (STATS) C.nodePops++;
+            }
+        }
+    }
+    return h;
+}
+
+// shadowRay (pathtrace.comp:211-242): unordered DFS with the 30-entry cap. Both
+// children are pushed untested; the pop tests the node's own box. Boxes of inner
+// nodes live in their parent, so each stack entry carries the box location.
+template <bool STATS>
+__device__ __forceinline__ bool unoccluded(const SceneView &S, v3 ro, v3 rd, float maxDist,
+                                           Counters &C) {
+    v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    if (STATS) C.shadowRays++;
+    // Entry encodes (parent inner index << 1 | side), or -1 for the root.
+    int stk[32];
+    int top = 0;
+    stk[top++] = -1;
+    while (top > 0) {
+        int e = stk[--top];
+        if (STATS) C.shadowPops++;
+        float mnx, mny, mnz, mxx, mxy, mxz;
+        int ref;
+        if (e < 0) {
+            mnx = S.rootMin[0]; mny = S.rootMin[1]; mnz = S.rootMin[2];
+            mxx = S.rootMax[0]; mxy = S.rootMax[1]; mxz = S.rootMax[2];
+            ref = S.rootRef;
+        } else {
+            const float4 *n = S.inner + 4 * (size_t)(e >> 1);
+            int4 r = ((const int4 *)n)[3];
+            if ((e & 1) == 0) {
+                float4 a = n[0], b = n[1];
+                mnx = a.x; mny = a.y; mnz = a.z; mxx = a.w; mxy = b.x; mxz = b.y;
+                ref = r.x;
+            } else {
+                float4 b = n[1], c = n[2];
+                mnx = b.z; mny = b.w; mnz = c.x; mxx = c.y; mxy = c.z; mxz = c.w;
+                ref = r.y;
+            }
+        }
+        float at;
+        if (!aabb(ro, inv, mnx, mny, mnz, mxx, mxy, mxz, at)) continue;
+        if (at > maxDist) continue;
+        if (ref < 0) {
+            int start, count;
+            leaf_range(S, ref, start, count);
+            for (int i = start; i < start + count; ++i) {
+                float t, bu, bv;
+                if (STATS) C.shadowTriTests++;
+                if (tri_test(S, ro, rd, i, t, bu, bv) && t < maxDist) return false;
+            }
+        } else if (top < 30) {
+            stk[top++] = (ref << 1) | 1;  // right pushed first
+            stk[top++] = (ref << 1);      // left popped first
+        }
+    }
+    return true;
+}
+
+}  // namespace rc

@@ -1,0 +1,81 @@
+// rc_internal.h — context object and kernel launch interfaces shared by the
+// translation units of libretrocam_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "rc_device.h"
+#include "retrocam_hip.h"
+
+namespace rc {
+
+// Device scene (see rc_device.h for the layout).
+struct DeviceScene {
+    float4 *inner = nullptr;
+    float4 *triPos = nullptr;
+    float4 *triShade = nullptr;
+    float4 *mats = nullptr;
+    float4 *lights = nullptr;
+    float *cdf = nullptr;
+    int *leafBigCount = nullptr;
+    int nTris = 0, nInner = 0, nLeaves = 0, nMats = 0, nLights = 0;
+    float totalLightPower = 0.0f;
+    float rootMin[3] = {0, 0, 0}, rootMax[3] = {0, 0, 0};
+    int rootRef = 0;
+    int maxDepth = 0;
+    bool valid = false;
+};
+
+struct Images {
+    int width = 0, height = 0;
+    float4 *accum = nullptr;       // RGBA32F
+    double4 *accum64 = nullptr;    // FP64 mode only
+    ushort4 *gbuf = nullptr;       // RGBA16F bits
+    ushort4 *galbedo = nullptr;    // RGBA16F bits
+    float *variance = nullptr;     // R32F
+    int *primId = nullptr;
+    int *gbufId = nullptr;
+};
+
+inline SceneView scene_view(const DeviceScene &D) {
+    SceneView S;
+    S.inner = D.inner;
+    S.triPos = D.triPos;
+    S.triShade = D.triShade;
+    S.mats = D.mats;
+    S.lights = D.lights;
+    S.cdf = D.cdf;
+    S.leafBigCount = D.leafBigCount;
+    for (int k = 0; k < 3; ++k) { S.rootMin[k] = D.rootMin[k]; S.rootMax[k] = D.rootMax[k]; }
+    S.rootRef = D.rootRef;
+    S.nLights = D.nLights;
+    S.totalLightPower = D.totalLightPower;
+    return S;
+}
+
+}  // namespace rc
+
+struct rc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    rc::DeviceScene scene;
+    rc::Images img;
+    int32_t totalSamples = 0;
+    int32_t accumMode = RC_ACCUM_FP32;
+    unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
+    int countersEnabled = 0;
+};
+
+namespace rc {
+
+int set_error(rc_ctx *ctx, int code, const std::string &msg);
+int hip_fail(rc_ctx *ctx, hipError_t e, const char *what);
+
+// Launches nSamples progressive passes starting at frame ctx->totalSamples.
+hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
+
+}  // namespace rc

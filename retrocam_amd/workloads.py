@@ -1,0 +1,150 @@
+"""Seeded synthetic workloads of the render hot path.
+
+  cornell_small   320x180, 16 spp, pinhole: 5 wall quads, 2 boxes, a ceiling
+                  quad light (radiance 15), a 32x16 UV sphere; Lambertian
+                  albedos U[0.1, 0.8] from the seed.
+  full_frame      854x480, 128 spp: ~250k triangles of jittered spheres/boxes
+                  (50% diffuse, 30% GGX metal alpha U[0.05, 0.6], 20% glass
+                  ior 1.5) in a closed room with 8 quad area lights; thin lens
+                  35 mm f/2.8, hexagon bokeh, focus 2.0 m, SA 0.04.
+  reference_default  the reference's default Cornell scene
+                  (src/main/java/com/retrocam/scene/SceneEditor.java:111-134) at 854x480.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .scene import Material, PtParams, SceneBuilder, Workload, lookat_camera, orbit_camera
+
+
+def cornell_small(seed: int = 1, width: int = 320, height: int = 180, spp: int = 16,
+                  clamp: float = 1.0) -> Workload:
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    alb = rng.uniform(0.1, 0.8, size=(8, 3)).astype(np.float32)
+    m = [b.add_material(Material.diffuse(*alb[i])) for i in range(8)]
+    light = b.add_material(Material.emissive(1.0, 1.0, 1.0, 15.0))
+    x0, x1, y0, y1, z0, z1 = -5.0, 5.0, 0.0, 10.0, -5.0, 5.0
+    b.add_quad((x0, y0, z0), (x0, y0, z1), (x1, y0, z1), (x1, y0, z0), m[0])  # floor
+    b.add_quad((x0, y1, z0), (x1, y1, z0), (x1, y1, z1), (x0, y1, z1), m[1])  # ceiling
+    b.add_quad((x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0), m[2])  # back
+    b.add_quad((x0, y0, z0), (x0, y1, z0), (x0, y1, z1), (x0, y0, z1), m[3])  # left
+    b.add_quad((x1, y0, z0), (x1, y0, z1), (x1, y1, z1), (x1, y1, z0), m[4])  # right
+    b.add_box((-1.5, 1.5, -1.0), (1.0, 1.5, 1.0), m[5])
+    b.add_box((1.5, 1.0, 0.5), (1.0, 1.0, 1.0), m[6])
+    ly = 9.99
+    b.add_quad((-1.5, ly, -1.5), (1.5, ly, -1.5), (1.5, ly, 1.5), (-1.5, ly, 1.5), light)
+    b.add_sphere((0.3, 1.2, -3.4), 1.2, m[7], stacks=16, slices=32)
+    scene = b.build()
+    cam = orbit_camera(width, height, pinhole=True)
+    return Workload("cornell_small", scene, cam, PtParams(1, clamp, 6), spp, {"seed": seed})
+
+
+def full_frame(seed: int = 1, n_tris: int = 250_000, width: int = 854, height: int = 480, spp: int = 128,
+               clamp: float = 1.0) -> Workload:
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    walls = [b.add_material(Material.diffuse(*rng.uniform(0.1, 0.8, 3))) for _ in range(6)]
+    lightm = b.add_material(Material.emissive(1.0, 0.95, 0.85, 12.0))
+    X, Y, Z = 8.0, 6.0, 8.0
+    # Room faces as 8x8 quad grids: large single triangles would share BVH nodes
+    # with dense object clusters and, with the builder's node-bounds binning
+    # (BVHBuilder.java:107-118), end up in leaves of hundreds of triangles.
+    g = 8
+
+    def grid(o, du, dv, mat):
+        for i in range(g):
+            for j in range(g):
+                a = [o[k] + du[k] * i / g + dv[k] * j / g for k in range(3)]
+                bb = [o[k] + du[k] * (i + 1) / g + dv[k] * j / g for k in range(3)]
+                c = [o[k] + du[k] * (i + 1) / g + dv[k] * (j + 1) / g for k in range(3)]
+                d = [o[k] + du[k] * i / g + dv[k] * (j + 1) / g for k in range(3)]
+                b.add_quad(a, bb, c, d, mat)
+
+    grid((-X, 0, -Z), (0, 0, 2 * Z), (2 * X, 0, 0), walls[0])
+    grid((-X, Y, -Z), (2 * X, 0, 0), (0, 0, 2 * Z), walls[1])
+    grid((-X, 0, -Z), (2 * X, 0, 0), (0, Y, 0), walls[2])
+    grid((-X, 0, Z), (0, Y, 0), (2 * X, 0, 0), walls[3])
+    grid((-X, 0, -Z), (0, Y, 0), (0, 0, 2 * Z), walls[4])
+    grid((X, 0, -Z), (0, 0, 2 * Z), (0, Y, 0), walls[5])
+    ly = Y - 0.01
+    for i in range(4):
+        for j in range(2):
+            cx, cz = -4.5 + 3.0 * i, -3.0 + 5.0 * j
+            b.add_quad((cx - 0.5, ly, cz - 0.5), (cx + 0.5, ly, cz - 0.5), (cx + 0.5, ly, cz + 0.5),
+                       (cx - 0.5, ly, cz + 0.5), lightm)
+    sphere_res = [(12, 24), (16, 32), (24, 48)]  # 552, 960, 2208 triangles (multiples of 12)
+    count = b.triangle_count()
+    assert (n_tris - count) % 12 == 0
+
+    def material():
+        u = rng.random()
+        if u < 0.5:
+            return b.add_material(Material.diffuse(*rng.uniform(0.1, 0.8, 3)))
+        if u < 0.8:
+            return b.add_material(Material.metal(*rng.uniform(0.5, 0.95, 3), float(rng.uniform(0.05, 0.6))))
+        return b.add_material(Material.glass(1.5))
+
+    while True:
+        remaining = n_tris - count
+        if remaining <= 0:
+            break
+        x, z = rng.uniform(-7.0, 7.0), rng.uniform(-7.0, 5.0)
+        r = float(rng.uniform(0.15, 0.6))
+        y = r + float(rng.uniform(0.0, 3.0)) * (rng.random() < 0.3)
+        if remaining >= 2208 and rng.random() < 0.6:
+            st, sl = sphere_res[int(rng.integers(0, 3))]
+            count += b.add_sphere((x, y, z), r, material(), stacks=st, slices=sl)
+        else:
+            rot = tuple(float(a) for a in rng.uniform(0.0, 90.0, 3))
+            count += b.add_box((x, y, z), (r, r * float(rng.uniform(0.5, 1.5)), r), material(), rot_deg=rot)
+    scene = b.build()
+    cam = lookat_camera(width, height, eye=(0.0, 2.2, 7.0), target=(0.0, 1.0, 0.0), focalDist=2.0,
+                        focalLengthMm=35.0, fStop=2.8, blades=6, saStrength=0.04)
+    return Workload("full_frame", scene, cam, PtParams(1, clamp, 6), spp, {"seed": seed, "n_tris": n_tris})
+
+
+def reference_default(width: int = 854, height: int = 480, spp: int = 16) -> Workload:
+    """SceneEditor.createDefault (SceneEditor.java:111-134), default orbit camera and lens."""
+    b = SceneBuilder()
+    white = b.add_material(Material.diffuse(0.73, 0.73, 0.73))
+    red = b.add_material(Material.diffuse(0.65, 0.05, 0.05))
+    green = b.add_material(Material.diffuse(0.12, 0.45, 0.15))
+    light = b.add_material(Material.emissive(1.0, 0.95, 0.85, 15.0))
+    metal = b.add_material(Material.metal(0.95, 0.85, 0.55, 0.1))
+    glass = b.add_material(Material.glass(1.5))
+    H = 5.0
+    b.add_box((0, 0, 0), (H, 0.05, H), white)
+    b.add_box((0, 2 * H, 0), (H, 0.05, H), white)
+    b.add_box((-H, H, 0), (0.05, H, H), red)
+    b.add_box((H, H, 0), (0.05, H, H), green)
+    b.add_box((0, H, -H), (H, H, 0.05), white)
+    b.add_box((0, 2 * H - 0.1, 0), (1.5, 0.05, 1.5), light)
+    b.add_box((-1.5, 1.5, -1.0), (1.0, 1.5, 1.0), white)
+    b.add_box((1.5, 1.0, 0.5), (1.0, 1.0, 1.0), metal)
+    b.add_sphere((0.0, 1.2, 1.5), 1.2, glass, stacks=24, slices=24)
+    scene = b.build()
+    cam = orbit_camera(width, height, focalDist=2.0, focalLengthMm=35.0, fStop=2.0, blades=5, saStrength=0.04)
+    return Workload("reference_default", scene, cam, PtParams(1, 1.0, 6), spp, {})
+
+
+def tiny_random(seed: int, n_tris: int = 48, extent: float = 2.0) -> "SceneArrays":
+    """Random triangle soup with random materials (for brute-force traversal tests)."""
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    mats = [b.add_material(Material.diffuse(0.5, 0.5, 0.5)),
+            b.add_material(Material.emissive(1.0, 1.0, 1.0, 5.0))]
+    tris = np.zeros((n_tris, 32), dtype=np.float32)
+    c = rng.uniform(-extent, extent, size=(n_tris, 3))
+    for k in range(3):
+        tris[:, 4 * k:4 * k + 3] = c + rng.uniform(-0.6, 0.6, size=(n_tris, 3))
+    e1 = tris[:, 4:7] - tris[:, 0:3]
+    e2 = tris[:, 8:11] - tris[:, 0:3]
+    n = np.cross(e1, e2)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-12)
+    for k in range(3):
+        tris[:, 12 + 4 * k:15 + 4 * k] = n
+    mi = rng.integers(0, len(mats), size=n_tris).astype(np.int32)
+    tris[:, 30] = mi.view(np.float32)
+    b.add_triangles(tris)
+    return b.build()
