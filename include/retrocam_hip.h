@@ -104,6 +104,45 @@ typedef struct rc_pt_params {
     int32_t maxBounces;     /* 6 in the reference (pathtrace.comp:465); 1..16     */
 } rc_pt_params;
 
+/* Post-chain settings: every RenderSettings field read by
+ * PostProcessStack.runChain (src/main/java/com/retrocam/post/PostProcessStack.java:140-280)
+ * and AtrousDenoiser.denoise (src/main/java/com/retrocam/post/AtrousDenoiser.java:60-125).
+ * Defaults are those of RenderSettings.java:26-191 (rc_post_defaults). */
+typedef struct rc_post_params {
+    float exposure;                   /* p00 EV multiplier                      */
+    int32_t atrousEnabled;            /* a-trous denoiser on/off                */
+    int32_t atrousIterations;         /* passes, clamped to 1..5                */
+    int32_t atrousMaxSpp;             /* bypass above this spp (0 = never)      */
+    float atrousSigmaColor, atrousSigmaNormal, atrousSigmaDepth;
+    int32_t enable[17];               /* enable[i] switches pass p<i>, i = 1..16;
+                                         p05/p06 are pass-through in the reference */
+    float lumaBlurSigma;                                                   /* p01 */
+    float chromaResSigma;                                                  /* p02 */
+    float colorTempBias, colorHueRot, colorSaturation, colorDriftSpeed;   /* p03 */
+    float drBlackLift, drShadowCrush, drHighlightKnee, drHighlightClip;   /* p04 */
+    float ccdNoiseLumaBase, ccdNoiseChromaScale;                          /* p07 */
+    float ccdSmearThreshold, ccdSmearIntensity; int32_t ccdSmearLength;   /* p08 */
+    float chromaBleedFactor;                                               /* p09 */
+    float timebaseAmplitudePx, timebaseFreq, timebaseSpeed;               /* p10 */
+    float dotCrawlIntensity, dotCrawlSubcarrier, dotCrawlEdgeThresh;      /* p11 */
+    float tapeAge, dropoutBright, dropoutBurstRate;                       /* p12 */
+    int32_t headSwitchLines;
+    float headSwitchJitter, headSwitchRipple, headSwitchChroma;           /* p13 */
+    float trackingSeverity, trackingMaxDisplacePx, trackingFringeWidthPx; /* p14 */
+    float edgeEnhanceAmount, edgeEnhanceCoreThreshold;                    /* p15 */
+    float interlaceFieldOffsetPx, interlaceCombStrength,
+          interlaceCombEdgeThresh, interlaceLineWeighting;                /* p16 */
+} rc_post_params;
+
+/* Time inputs of the post chain: TemporalState.time / agcGain
+ * (src/main/java/com/retrocam/core/TemporalState.java:96-111) and
+ * RenderSettings.frameIndex (incremented per displayed frame, Main.java:313). */
+typedef struct rc_temporal {
+    float time;
+    float agcGain;
+    int32_t frameIndex;
+} rc_temporal;
+
 /* Readable buffers for rc_read. */
 enum {
     RC_BUF_ACCUM = 0,       /* float4 per px: sum of radiance, sample count (RGBA32F) */
@@ -114,7 +153,9 @@ enum {
                                hit in the most recent pass (-1 = miss); triangles are
                                numbered in upload (BVH leaf) order                 */
     RC_BUF_GBUFFER_ID = 5,  /* int32 per px: triangle index hit by the G-buffer ray */
-    RC_BUF_ACCUM64 = 6      /* double4 per px: FP64 accumulation (RC_ACCUM_FP64)   */
+    RC_BUF_ACCUM64 = 6,     /* double4 per px: FP64 accumulation (RC_ACCUM_FP64)   */
+    RC_BUF_POST_HDR = 7,    /* float4 per px: post-chain output before display     */
+    RC_BUF_DISPLAY = 8      /* uint8x4 per px: display-encoded RGBA8 (ACES + sRGB)  */
 };
 
 /* Accumulation precision (rc_set_accum_mode). */
@@ -169,6 +210,21 @@ int rc_set_accum_mode(rc_ctx *ctx, int32_t mode);
 int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes);
 /* Size in bytes of a readable buffer at the current resolution. */
 size_t rc_buffer_bytes(const rc_ctx *ctx, int32_t which);
+
+/* ---- post chain ---------------------------------------------------------- */
+/* Fills p with the RenderSettings defaults (RenderSettings.java:26-191):
+ * a-trous off (4 iterations), every VHS pass off. */
+void rc_post_defaults(rc_post_params *p);
+/* PostProcessStack.runOnAccum + the display pass (PostProcessStack.java:109-118,
+ * RenderPipeline.java:228-261) on the context's accumulation: p00 divides by the
+ * total sample count and applies p->exposure, then the a-trous passes (if
+ * enabled and the count is within atrousMaxSpp), the enabled VHS passes p01..p16
+ * in order, and the display encode. Read RC_BUF_POST_HDR / RC_BUF_DISPLAY. */
+int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t);
+/* PostProcessStack.runOnImage (PostProcessStack.java:130-136): the chain on a
+ * host RGBA32F image (rows bottom-up) with N = 1, exposure 1 and no a-trous. */
+int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
+                  const rc_temporal *t);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Work counters of the path-trace kernel, for the roofline byte model. Copies

@@ -80,6 +80,26 @@ int32_t rco_shadow_ray(const rco_scene *s, const float *ro, const float *rd, flo
 int32_t rco_render_pass(const rco_scene *s, const rc_camera *cam, const rc_pt_params *p,
                         int32_t frameIndex, int32_t pxStride, rco_frame *out, rco_stats *stats);
 
+/* Post chain of one frame (rc_oracle_post.c): p00 normalize of the accumulation
+ * sum, the a-trous passes, the enabled VHS passes p01..p16 (PostProcessStack.
+ * runChain, src/main/java/com/retrocam/post/PostProcessStack.java:109-280) and the
+ * display encode (display.frag with totalSamples = exposure = 1, then RGBA8).
+ * gbuf/galbedo/variance may be NULL (then no a-trous: runOnImage). outHdr
+ * (float4) and outRgba8 (uint8 x4) may be NULL. Returns 0 or -1. */
+int32_t rco_post_chain(const float *accum, const uint16_t *gbuf, const uint16_t *galbedo, const float *variance,
+                       int32_t W, int32_t H, int32_t totalSamples, const rc_post_params *p, const rc_temporal *t,
+                       float *outHdr, uint8_t *outRgba8);
+/* One pass over a float4 image: 0 = p00, 1..16 = VHS pass pN (5, 6 = copy),
+ * 100 + i = a-trous pass i (step 1 << i), 200 = display encode (float, pre-8-bit). */
+int32_t rco_post_pass(int32_t pass, const float *src, float *dst, int32_t W, int32_t H, const rc_post_params *p,
+                      const rc_temporal *t, const uint16_t *gbuf, const uint16_t *galbedo, const float *variance,
+                      int32_t totalSamples);
+
+/* VHS integer hashes: kind 0 pcgHash (p07_ccd_noise.frag:50-54), 1 one-round
+ * uHash (p12_dropout.frag:32-37), 2 two-round uHash (p13_headswitch.frag:40-45),
+ * 3 / 4 the float bits of fHash built on 1 / 2. */
+uint32_t rco_hash(int32_t kind, uint32_t a, uint32_t b);
+
 /* Element-wise rc_fmath routines: fn 0 sin, 1 cos, 2 exp, 3 log, 4 pow(x, y),
  * 5 binary32 -> binary16 (RNE) -> binary32. Returns 0, or -1 for a bad fn. */
 int32_t rco_fmath(int32_t fn, const float *x, const float *y, float *out, int32_t n);

@@ -72,6 +72,15 @@ def lib() -> C.CDLL:
         L.rco_render_pass.restype = C.c_int32
         L.rco_render_pass.argtypes = [C.POINTER(RcoScene), C.POINTER(RcCamera), C.POINTER(RcPtParams),
                                       C.c_int32, C.c_int32, C.POINTER(RcoFrame), C.POINTER(RcoStats)]
+        from retrocam_amd._native import RcPostParams, RcTemporal
+        L.rco_post_chain.restype = C.c_int32
+        L.rco_post_chain.argtypes = [_FP, _VP, _VP, _FP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
+                                     C.POINTER(RcTemporal), _FP, _VP]
+        L.rco_post_pass.restype = C.c_int32
+        L.rco_post_pass.argtypes = [C.c_int32, _FP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
+                                    C.POINTER(RcTemporal), _VP, _VP, _FP, C.c_int32]
+        L.rco_hash.restype = C.c_uint32
+        L.rco_hash.argtypes = [C.c_int32, C.c_uint32, C.c_uint32]
         L.rco_fmath.restype = C.c_int32
         L.rco_fmath.argtypes = [C.c_int32, _FP, _FP, _FP, C.c_int32]
         L.rco_num_threads.restype = C.c_int32
@@ -137,6 +146,40 @@ class OracleRenderer:
         m = np.zeros((H, W), dtype=bool)
         m[::self.stride, ::self.stride] = True
         return m.reshape(-1)
+
+
+def post_chain(accum, W, H, total, params, temporal, gbuf=None, galbedo=None, variance=None):
+    """rco_post_chain: returns (hdr float32 (H*W,4), rgba8 uint8 (H*W,4))."""
+    accum = np.ascontiguousarray(accum, np.float32).reshape(-1, 4)
+    hdr = np.zeros((W * H, 4), np.float32)
+    rgba = np.zeros((W * H, 4), np.uint8)
+    gb = None if gbuf is None else np.ascontiguousarray(gbuf, np.uint16)
+    ga = None if galbedo is None else np.ascontiguousarray(galbedo, np.uint16)
+    va = None if variance is None else np.ascontiguousarray(variance, np.float32)
+    rc = lib().rco_post_chain(_fp(accum), _vp(gb), _vp(ga), None if va is None else _fp(va), W, H, int(total),
+                              C.byref(params), C.byref(temporal), _fp(hdr), rgba.ctypes.data_as(_VP))
+    if rc != 0:
+        raise RuntimeError("rco_post_chain failed")
+    return hdr, rgba
+
+
+def post_pass(pass_id, src, W, H, params, temporal, gbuf=None, galbedo=None, variance=None, total=1):
+    src = np.ascontiguousarray(src, np.float32).reshape(-1, 4)
+    dst = np.zeros_like(src)
+    gb = None if gbuf is None else np.ascontiguousarray(gbuf, np.uint16)
+    ga = None if galbedo is None else np.ascontiguousarray(galbedo, np.uint16)
+    va = None if variance is None else np.ascontiguousarray(variance, np.float32)
+    rc = lib().rco_post_pass(int(pass_id), _fp(src), _fp(dst), W, H, C.byref(params), C.byref(temporal), _vp(gb),
+                             _vp(ga), None if va is None else _fp(va), int(total))
+    if rc != 0:
+        raise RuntimeError("rco_post_pass failed")
+    return dst
+
+
+def hash_u32(kind: int, a: int, b: int = 0) -> int:
+    """0 pcgHash(a), 1 one-round uHash(a), 2 two-round uHash(a),
+    3 float bits of fHash1(a, b), 4 float bits of fHash2(a, b)."""
+    return int(lib().rco_hash(int(kind), int(a) & 0xFFFFFFFF, int(b) & 0xFFFFFFFF))
 
 
 def fmath(fn: int, x, y=None):

@@ -30,6 +30,8 @@ RC_BUF_VARIANCE = 3
 RC_BUF_PRIMARY_ID = 4
 RC_BUF_GBUFFER_ID = 5
 RC_BUF_ACCUM64 = 6
+RC_BUF_POST_HDR = 7
+RC_BUF_DISPLAY = 8
 
 RC_ACCUM_FP32 = 0
 RC_ACCUM_FP64 = 1
@@ -48,6 +50,36 @@ class RcCamera(C.Structure):
 
 class RcPtParams(C.Structure):
     _fields_ = [("neeEnabled", C.c_int32), ("neeFireflyClamp", C.c_float), ("maxBounces", C.c_int32)]
+
+
+_F = C.c_float
+_I = C.c_int32
+
+
+class RcPostParams(C.Structure):
+    _fields_ = [
+        ("exposure", _F), ("atrousEnabled", _I), ("atrousIterations", _I), ("atrousMaxSpp", _I),
+        ("atrousSigmaColor", _F), ("atrousSigmaNormal", _F), ("atrousSigmaDepth", _F),
+        ("enable", _I * 17),
+        ("lumaBlurSigma", _F), ("chromaResSigma", _F),
+        ("colorTempBias", _F), ("colorHueRot", _F), ("colorSaturation", _F), ("colorDriftSpeed", _F),
+        ("drBlackLift", _F), ("drShadowCrush", _F), ("drHighlightKnee", _F), ("drHighlightClip", _F),
+        ("ccdNoiseLumaBase", _F), ("ccdNoiseChromaScale", _F),
+        ("ccdSmearThreshold", _F), ("ccdSmearIntensity", _F), ("ccdSmearLength", _I),
+        ("chromaBleedFactor", _F),
+        ("timebaseAmplitudePx", _F), ("timebaseFreq", _F), ("timebaseSpeed", _F),
+        ("dotCrawlIntensity", _F), ("dotCrawlSubcarrier", _F), ("dotCrawlEdgeThresh", _F),
+        ("tapeAge", _F), ("dropoutBright", _F), ("dropoutBurstRate", _F),
+        ("headSwitchLines", _I), ("headSwitchJitter", _F), ("headSwitchRipple", _F), ("headSwitchChroma", _F),
+        ("trackingSeverity", _F), ("trackingMaxDisplacePx", _F), ("trackingFringeWidthPx", _F),
+        ("edgeEnhanceAmount", _F), ("edgeEnhanceCoreThreshold", _F),
+        ("interlaceFieldOffsetPx", _F), ("interlaceCombStrength", _F), ("interlaceCombEdgeThresh", _F),
+        ("interlaceLineWeighting", _F),
+    ]
+
+
+class RcTemporal(C.Structure):
+    _fields_ = [("time", _F), ("agcGain", _F), ("frameIndex", _I)]
 
 
 class RcsInfo(C.Structure):
@@ -77,6 +109,9 @@ HIP_SYMBOLS = {
     "rc_set_accum_mode": (C.c_int, [_VP, C.c_int32]),
     "rc_read": (C.c_int, [_VP, C.c_int32, _VP, C.c_size_t]),
     "rc_buffer_bytes": (C.c_size_t, [_VP, C.c_int32]),
+    "rc_post_defaults": (None, [C.POINTER(RcPostParams)]),
+    "rc_post": (C.c_int, [_VP, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
+    "rc_post_image": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
     "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),
     "rc_debug_pcg": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
     "rc_debug_rays": (C.c_int, [_VP, _FP, C.c_int32, C.POINTER(C.c_int32), _FP, C.POINTER(C.c_int32)]),

@@ -41,7 +41,69 @@ static void free_scene(DeviceScene &d) {
 static void free_images(Images &im) {
     dfree(im.accum); dfree(im.accum64); dfree(im.gbuf); dfree(im.galbedo);
     dfree(im.variance); dfree(im.primId); dfree(im.gbufId);
+    dfree(im.postA); dfree(im.postB); dfree(im.postIn); dfree(im.display);
     im = Images{};
+}
+
+template <typename T>
+static int dalloc(rc_ctx *ctx, T *&p, size_t count, const char *what);
+
+// Post buffers for a w x h chain (kept until the resolution changes).
+static int ensure_post(rc_ctx *ctx, int w, int h) {
+    Images &im = ctx->img;
+    if (im.postW == w && im.postH == h && im.postA) return RC_OK;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    dfree(im.postA); dfree(im.postB); dfree(im.postIn); dfree(im.display);
+    im.postOut = nullptr;
+    im.postW = im.postH = 0;
+    size_t n = (size_t)w * (size_t)h;
+    int rc;
+    if ((rc = dalloc(ctx, im.postA, n, "hipMalloc(postA)"))) return rc;
+    if ((rc = dalloc(ctx, im.postB, n, "hipMalloc(postB)"))) return rc;
+    if ((rc = dalloc(ctx, im.postIn, n, "hipMalloc(postIn)"))) return rc;
+    if ((rc = dalloc(ctx, im.display, n, "hipMalloc(display)"))) return rc;
+    im.postW = w;
+    im.postH = h;
+    return RC_OK;
+}
+
+// PostProcessStack.runChain (PostProcessStack.java:140-280) from the p00 input
+// `src` (accumulation sum, or an image with total = 1), then the display pass.
+static int run_post_chain(rc_ctx *ctx, const float4 *src, int w, int h, const rc_post_params &p,
+                          const rc_temporal &t, int total, bool withGbuf) {
+    Images &im = ctx->img;
+    float4 *a = im.postA, *b = im.postB;
+    hipError_t e = launch_post_pass(ctx->stream, 0, src, a, w, h, p, t, im.gbuf, im.galbedo, im.variance, total, 0,
+                                    nullptr);
+    if (e != hipSuccess) return hip_fail(ctx, e, "post p00 launch");
+    bool exceeded = p.atrousMaxSpp > 0 && total > p.atrousMaxSpp;
+    if (p.atrousEnabled && withGbuf && !exceeded) {
+        int np = p.atrousIterations < 1 ? 1 : (p.atrousIterations > 5 ? 5 : p.atrousIterations);
+        for (int i = 0; i < np; ++i) {  // AtrousDenoiser.denoise (AtrousDenoiser.java:60-125)
+            e = launch_post_pass(ctx->stream, 100, a, b, w, h, p, t, im.gbuf, im.galbedo, im.variance, total, 1 << i,
+                                 nullptr);
+            if (e != hipSuccess) return hip_fail(ctx, e, "post a-trous launch");
+            float4 *tmp = a; a = b; b = tmp;
+        }
+    }
+    for (int k = 1; k <= 16; ++k) {
+        if (k == 5 || k == 6 || !p.enable[k]) continue;  // p05/p06 pass through (PostProcessStack.java:184)
+        e = launch_post_pass(ctx->stream, k, a, b, w, h, p, t, nullptr, nullptr, nullptr, total, 0, nullptr);
+        if (e != hipSuccess) return hip_fail(ctx, e, "post pass launch");
+        float4 *tmp = a; a = b; b = tmp;
+    }
+    e = launch_post_pass(ctx->stream, 200, a, nullptr, w, h, p, t, nullptr, nullptr, nullptr, 1, 0, im.display);
+    if (e != hipSuccess) return hip_fail(ctx, e, "display launch");
+    im.postOut = a;
+    return RC_OK;
+}
+
+static int validate_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t) {
+    if (!p || !t) return set_error(ctx, RC_ERR_INVALID_ARG, "post params or temporal state is NULL");
+    if (p->ccdSmearLength < 0 || p->headSwitchLines < 0)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "ccdSmearLength and headSwitchLines must be >= 0");
+    return RC_OK;
 }
 
 template <typename T>
@@ -390,8 +452,97 @@ size_t rc_buffer_bytes(const rc_ctx *ctx, int32_t which) {
         case RC_BUF_PRIMARY_ID: return n * 4;
         case RC_BUF_GBUFFER_ID: return n * 4;
         case RC_BUF_ACCUM64: return ctx->img.accum64 ? n * 32 : 0;
+        case RC_BUF_POST_HDR:
+            return ctx->img.postOut ? (size_t)ctx->img.postW * (size_t)ctx->img.postH * 16 : 0;
+        case RC_BUF_DISPLAY:
+            return ctx->img.postOut ? (size_t)ctx->img.postW * (size_t)ctx->img.postH * 4 : 0;
         default: return 0;
     }
+}
+
+void rc_post_defaults(rc_post_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // RenderSettings.java:26-191 defaults
+    p->exposure = 1.0f;
+    p->atrousEnabled = 0;
+    p->atrousIterations = 4;
+    p->atrousMaxSpp = 0;
+    p->atrousSigmaColor = 1.0f;
+    p->atrousSigmaNormal = 64.0f;
+    p->atrousSigmaDepth = 0.15f;
+    p->lumaBlurSigma = 1.5f;
+    p->chromaResSigma = 7.0f;
+    p->colorTempBias = 0.0f;
+    p->colorHueRot = 0.0f;
+    p->colorSaturation = 0.95f;
+    p->colorDriftSpeed = 0.05f;
+    p->drBlackLift = 0.01f;
+    p->drShadowCrush = 0.03f;
+    p->drHighlightKnee = 0.78f;
+    p->drHighlightClip = 0.95f;
+    p->ccdNoiseLumaBase = 0.015f;
+    p->ccdNoiseChromaScale = 1.6f;
+    p->ccdSmearThreshold = 0.85f;
+    p->ccdSmearIntensity = 0.009f;
+    p->ccdSmearLength = 20;
+    p->chromaBleedFactor = 0.35f;
+    p->timebaseAmplitudePx = 1.5f;
+    p->timebaseFreq = 0.02f;
+    p->timebaseSpeed = 0.70f;
+    p->dotCrawlIntensity = 0.25f;
+    p->dotCrawlSubcarrier = 0.14f;
+    p->dotCrawlEdgeThresh = 0.04f;
+    p->tapeAge = 0.3f;
+    p->dropoutBright = 0.92f;
+    p->dropoutBurstRate = 0.45f;
+    p->headSwitchLines = 12;
+    p->headSwitchJitter = 4.0f;
+    p->headSwitchRipple = 0.10f;
+    p->headSwitchChroma = 0.35f;
+    p->trackingSeverity = 0.1f;
+    p->trackingMaxDisplacePx = 28.0f;
+    p->trackingFringeWidthPx = 4.0f;
+    p->edgeEnhanceAmount = 1.5f;
+    p->edgeEnhanceCoreThreshold = 0.025f;
+    p->interlaceFieldOffsetPx = 0.5f;
+    p->interlaceCombStrength = 1.5f;
+    p->interlaceCombEdgeThresh = 0.04f;
+    p->interlaceLineWeighting = 0.05f;
+}
+
+int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = validate_post(ctx, p, t))) return rc;
+    if (ctx->img.width <= 0 || ctx->totalSamples <= 0)
+        return set_error(ctx, RC_ERR_STATE, "nothing accumulated: render before rc_post");
+    (void)hipSetDevice(ctx->device);
+    if ((rc = ensure_post(ctx, ctx->img.width, ctx->img.height))) return rc;
+    return run_post_chain(ctx, ctx->img.accum, ctx->img.width, ctx->img.height, *p, *t, ctx->totalSamples, true);
+}
+
+int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
+                  const rc_temporal *t) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    int rc;
+    if ((rc = validate_post(ctx, p, t))) return rc;
+    if (!rgba || width <= 0 || height <= 0 || width > 16384 || height > 16384)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "image pointer/size invalid");
+    (void)hipSetDevice(ctx->device);
+    if ((rc = ensure_post(ctx, width, height))) return rc;
+    hipError_t e = hipMemcpyAsync(ctx->img.postIn, rgba, (size_t)width * (size_t)height * 16, hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "upload(image)");
+    // PostProcessStack.runOnImage (PostProcessStack.java:130-136): p00 with N = 1 and
+    // exposure 1, no a-trous (no G-buffer).
+    rc_post_params q = *p;
+    q.exposure = 1.0f;
+    rc = run_post_chain(ctx, ctx->img.postIn, width, height, q, *t, 1, false);
+    if (rc) return rc;
+    e = hipStreamSynchronize(ctx->stream);  // the host image may be freed on return
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    return RC_OK;
 }
 
 int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes) {
@@ -409,6 +560,8 @@ int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes) {
         case RC_BUF_PRIMARY_ID: src = ctx->img.primId; break;
         case RC_BUF_GBUFFER_ID: src = ctx->img.gbufId; break;
         case RC_BUF_ACCUM64: src = ctx->img.accum64; break;
+        case RC_BUF_POST_HDR: src = ctx->img.postOut; break;
+        case RC_BUF_DISPLAY: src = ctx->img.display; break;
     }
     (void)hipSetDevice(ctx->device);
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);

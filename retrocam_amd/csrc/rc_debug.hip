@@ -17,15 +17,17 @@ __global__ void pcg_stream_kernel(const int *__restrict__ xyf, int n, int count,
 // rays: 8 floats each {ro.xyz, maxDist, rd.xyz, -}
 __global__ void ray_query_kernel(SceneView S, const float4 *__restrict__ rays, int n, int *__restrict__ tri,
                                  float *__restrict__ t, int *__restrict__ unocc) {
+    __shared__ int2 stk[kShortStack][kBlock];
+    StackRef st{stk, (int)threadIdx.x};
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     v3 ro = mk(a.x, a.y, a.z), rd = mk(b.x, b.y, b.z);
     Counters C;
-    Hit h = closest_hit<false>(S, ro, rd, C);
+    Hit h = closest_hit<false>(S, ro, rd, st, C);
     tri[i] = h.tri;
     t[i] = h.t;
-    unocc[i] = unoccluded<false>(S, ro, rd, a.w, C) ? 1 : 0;
+    unocc[i] = unoccluded<false>(S, ro, rd, a.w, st, C) ? 1 : 0;
 }
 
 }  // namespace rc

@@ -134,12 +134,24 @@ struct Hit {
     int tri;
 };
 
+// Traversal stacks: the first kShortStack entries of each lane live in LDS
+// (row-major [entry][lane] so a wave's accesses hit 64 distinct banks); deeper
+// entries spill to a private array. The logical stack depth and the reference's
+// caps (62 closest-hit, 30 shadow) are unchanged by where entries live.
+constexpr int kShortStack = 12;
+constexpr int kBlock = 256;
+
+struct StackRef {
+    int2 (*lds)[kBlock];  // __shared__ int2[kShortStack][kBlock]
+    int lane;             // threadIdx.x
+};
+
 // traverseBVH (pathtrace.comp:142-203). The stack holds (ref, tEntry): the child
 // box test at push time is the same computation as the reference's re-test of the
 // node's own box at pop, so the pop-time cull `aabbT > rec.t` reads tEntry.
-// The 62-entry cap (pathtrace.comp:176) is applied identically.
+// The 62-entry cap (pathtrace.comp:176) is applied to the same logical depth.
 template <bool STATS>
-__device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Counters &C) {
+__device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, StackRef st, Counters &C) {
     Hit h;
     h.t = 1e30f; h.tri = -1; h.bu = 0.0f; h.bv = 0.0f;
     v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
@@ -148,8 +160,7 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Cou
     if (!aabb(ro, inv, S.rootMin[0], S.rootMin[1], S.rootMin[2], S.rootMax[0], S.rootMax[1],
               S.rootMax[2], tr))
         return h;
-    int stkRef[64];
-    float stkT[64];
+    int2 spill[64 - kShortStack];
     int top = 0;
     int ref = S.rootRef;
     float tEntry = tr;
@@ -158,8 +169,9 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Cou
         if (!have) {
             if (top == 0) break;
             --top;
-            ref = stkRef[top];
-            tEntry = stkT[top];
+            int2 e = top < kShortStack ? st.lds[top][st.lane] : spill[top - kShortStack];
+            ref = e.x;
+            tEntry = __int_as_float(e.y);
             if (STATS) C.nodePops++;
         }
         have = false;
@@ -183,22 +195,21 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Cou
             float tl, trr;
             bool hl = aabb(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
             bool hr = aabb(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, trr);
-            if (hl && hr) {
-                if (tl < trr) {
-                    stkRef[top] = r.y; stkT[top] = trr; ++top;
-                    ref = r.x; tEntry = tl;
-                } else {
-                    stkRef[top] = r.x; stkT[top] = tl; ++top;
-                    ref = r.y; tEntry = trr;
-                }
+            if (hl || hr) {
+                if (STATS) C.nodePops++;
                 have = true;  // the entry the reference pushes last is popped next
-                if (STATS) C.nodePops++;
-            } else if (hl) {
-                ref = r.x; tEntry = tl; have = true;
-                if (STATS) C.nodePops++;
-            } else if (hr) {
-                ref = r.y; tEntry = trr; have = true;
-                if (STATS) C.nodePops++;
+                if (hl && hr) {
+                    bool nearLeft = tl < trr;
+                    int2 far = nearLeft ? make_int2(r.y, __float_as_int(trr)) : make_int2(r.x, __float_as_int(tl));
+                    if (top < kShortStack) st.lds[top][st.lane] = far;
+                    else spill[top - kShortStack] = far;
+                    ++top;
+                    ref = nearLeft ? r.x : r.y;
+                    tEntry = nearLeft ? tl : trr;
+                } else {
+                    ref = hl ? r.x : r.y;
+                    tEntry = hl ? tl : trr;
+                }
             }
         }
     }
@@ -207,18 +218,24 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Cou
 
 // shadowRay (pathtrace.comp:211-242): unordered DFS with the 30-entry cap. Both
 // children are pushed untested; the pop tests the node's own box. Boxes of inner
-// nodes live in their parent, so each stack entry carries the box location.
+// nodes live in their parent, so each stack entry names (parent << 1 | side);
+// -1 is the root.
 template <bool STATS>
-__device__ __forceinline__ bool unoccluded(const SceneView &S, v3 ro, v3 rd, float maxDist,
+__device__ __forceinline__ bool unoccluded(const SceneView &S, v3 ro, v3 rd, float maxDist, StackRef st,
                                            Counters &C) {
     v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
     if (STATS) C.shadowRays++;
-    // Entry encodes (parent inner index << 1 | side), or -1 for the root.
-    int stk[32];
+    int spill[32 - kShortStack];
     int top = 0;
-    stk[top++] = -1;
-    while (top > 0) {
-        int e = stk[--top];
+    int e = -1;
+    bool have = true;
+    while (true) {
+        if (!have) {
+            if (top == 0) break;
+            --top;
+            e = top < kShortStack ? st.lds[top][st.lane].x : spill[top - kShortStack];
+        }
+        have = false;
         if (STATS) C.shadowPops++;
         float mnx, mny, mnz, mxx, mxy, mxz;
         int ref;
@@ -251,8 +268,13 @@ __device__ __forceinline__ bool unoccluded(const SceneView &S, v3 ro, v3 rd, flo
                 if (tri_test(S, ro, rd, i, t, bu, bv) && t < maxDist) return false;
             }
         } else if (top < 30) {
-            stk[top++] = (ref << 1) | 1;  // right pushed first
-            stk[top++] = (ref << 1);      // left popped first
+            // push right, then left; the left entry is popped next
+            int right = (ref << 1) | 1;
+            if (top < kShortStack) st.lds[top][st.lane].x = right;
+            else spill[top - kShortStack] = right;
+            ++top;
+            e = ref << 1;
+            have = true;
         }
     }
     return true;

@@ -12,13 +12,13 @@
  *
  * Methods (published, restated here):
  *   sin/cos — Cody & Waite (1980) additive range reduction by pi/2 with a
- *             three-part constant, then the minimax polynomials of the Cephes
+ *             four-part constant, then the minimax polynomials of the Cephes
  *             library's sinf/cosf (S. L. Moshier, 1984-1992) on [-pi/4, pi/4].
  *   exp     — Cody-Waite reduction by ln 2, Cephes expf polynomial, ldexpf.
  *   log     — frexpf + Cephes logf polynomial on [sqrt(1/2)-1, sqrt(2)-1].
  *   pow     — exp(y * log(x)) for x > 0 (the GLSL definition for x > 0).
  * Accuracy (checked against float64 in the tests): sin/cos <= 2 ulp plus
- * 2^-24 absolute for |x| <= 2000; exp <= 2 ulp on [-87, 88]; log <= 2 ulp.
+ * 2^-24 absolute for |x| <= 1e5; exp <= 2 ulp on [-87, 88]; log <= 2 ulp.
  */
 #ifndef RC_FMATH_H
 #define RC_FMATH_H
@@ -38,11 +38,12 @@ RC_FN uint32_t rc_f2u(float f) { union { float f; uint32_t u; } c; c.f = f; retu
 RC_FN float rc_u2f(uint32_t u) { union { float f; uint32_t u; } c; c.u = u; return c.f; }
 RC_FN int32_t rc_f2i_bits(float f) { union { float f; int32_t i; } c; c.f = f; return c.i; }
 
-/* pi/2 = RC_PIO2_HI + RC_PIO2_MID + RC_PIO2_LO; HI and MID carry 12 significant
- * bits each so k*HI and k*MID are exact for |k| < 2^12. */
-#define RC_PIO2_HI   1.57080078125f
-#define RC_PIO2_MID (-4.453584551811218e-06f)
-#define RC_PIO2_LO  (-8.705516307827565e-10f)
+/* pi/2 = P1 + P2 + P3 + P4; P1..P3 carry 8 significant bits each, so k*P1,
+ * k*P2 and k*P3 are exact for |k| < 2^16 (|x| < ~1e5). */
+#define RC_PIO2_P1  1.5703125f
+#define RC_PIO2_P2  0.000484466552734375f
+#define RC_PIO2_P3 (-6.407499313354492e-07f)
+#define RC_PIO2_P4  9.92093629470503e-10f
 #define RC_2_OVER_PI 0.6366197466850281f
 
 RC_FN float rc_poly_sin(float r) {
@@ -64,9 +65,10 @@ RC_FN float rc_poly_cos(float r) {
 /* Reduces x to r in [-pi/4, pi/4] and quadrant q (x = r + q*pi/2, mod 4). */
 RC_FN float rc_reduce_pio2(float x, int *q) {
     float k = rintf(x * RC_2_OVER_PI);
-    float r = x - k * RC_PIO2_HI;
-    r = r - k * RC_PIO2_MID;
-    r = r - k * RC_PIO2_LO;
+    float r = x - k * RC_PIO2_P1;
+    r = r - k * RC_PIO2_P2;
+    r = r - k * RC_PIO2_P3;
+    r = r - k * RC_PIO2_P4;
     *q = ((int)k) & 3;
     return r;
 }

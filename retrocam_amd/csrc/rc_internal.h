@@ -38,6 +38,12 @@ struct Images {
     float *variance = nullptr;     // R32F
     int *primId = nullptr;
     int *gbufId = nullptr;
+    // post chain (allocated on first rc_post / rc_post_image)
+    int postW = 0, postH = 0;
+    float4 *postA = nullptr, *postB = nullptr;  // RGBA32F ping-pong
+    float4 *postIn = nullptr;                   // rc_post_image input
+    uchar4 *display = nullptr;                  // RGBA8 display output
+    const float4 *postOut = nullptr;            // which of postA/postB holds the last chain output
 };
 
 inline SceneView scene_view(const DeviceScene &D) {
@@ -77,5 +83,11 @@ int hip_fail(rc_ctx *ctx, hipError_t e, const char *what);
 
 // Launches nSamples progressive passes starting at frame ctx->totalSamples.
 hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
+
+// One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
+// with the given step, 200 = display encode into rgba8.
+hipError_t launch_post_pass(hipStream_t stream, int pass, const float4 *src, float4 *dst, int w, int h,
+                            const rc_post_params &p, const rc_temporal &t, const ushort4 *gbuf,
+                            const ushort4 *galbedo, const float *variance, int total, int step, uchar4 *rgba8);
 
 }  // namespace rc

@@ -2,9 +2,16 @@
 //
 // One work-item per pixel; a launch runs nSamples consecutive progressive
 // passes (frame indices f0 .. f0+nSamples-1) and keeps the running sums in
-// registers, so the accumulation buffers are read and written once per launch
-// instead of once per sample. The per-sample arithmetic restates
-// pathtrace.comp:401-655 (raygen, traceRadiance, NEE, accumulate) exactly.
+// registers, so the accumulation buffers are read and written once per launch.
+// The per-sample arithmetic restates pathtrace.comp:401-655 (raygen,
+// traceRadiance, NEE, accumulate) with the same operation order.
+//
+// Scheduling: each lane owns one pixel and regenerates its path state as soon
+// as a path ends (next LCA channel, else next sample), so the lanes of a wave
+// stay busy until every lane has finished all its samples, instead of waiting at
+// each sample for the longest path of the wave. Samples of a pixel are still
+// accumulated in frame order, so every sum is the sequential one of the
+// reference. Traversal stacks live in LDS (rc_device.h).
 //
 // Launch shape: 256-thread workgroups covering 16x16 pixel tiles, each wave64
 // an 8x8 sub-tile so camera rays of one wave are spatially coherent.
@@ -28,7 +35,6 @@ struct PassArgs {
     int maxBounces;
     int frame0;
     int nSamples;
-    int fp64;
     float4 *accum;
     double4 *accum64;
     ushort4 *gbuf;
@@ -115,163 +121,13 @@ __device__ __forceinline__ void gen_ray(const CamArgs &c, int px, int py, uint32
     rd = normalize(fp - o);
 }
 
-struct HitInfo {
-    bool valid;
-    float t;
-    v3 pos, normal;
-    int mat, tri;
-};
-
-template <bool STATS>
-__device__ __forceinline__ HitInfo trace_closest(const SceneView &S, v3 ro, v3 rd, Counters &C) {
-    Hit h = closest_hit<STATS>(S, ro, rd, C);
-    HitInfo r;
-    r.valid = h.tri >= 0;
-    r.t = h.t;
-    r.tri = h.tri;
-    r.mat = 0;
-    r.pos = mk(0, 0, 0);
-    r.normal = mk(0, 0, 0);
-    if (r.valid) {
-        const float4 *sh = S.triShade + 3 * (size_t)h.tri;
-        float4 a = sh[0], b = sh[1], c = sh[2];
-        float w = (1.0f - h.bu) - h.bv;
-        r.normal = normalize((xyz(a) * w + xyz(b) * h.bu) + xyz(c) * h.bv);
-        r.mat = rc_f2i_bits(a.w);
-        r.pos = ro + rd * h.t;
-    }
-    return r;
-}
-
-// sampleDirectLight (pathtrace.comp:341-395).
-template <bool STATS>
-__device__ __forceinline__ v3 direct_light(const PassArgs &A, v3 hp, v3 n, v3 albedo, uint32_t &seed,
-                                           Counters &C) {
-    const SceneView &S = A.S;
-    if (S.nLights <= 0 || S.totalLightPower <= 0.0f) return mk(0, 0, 0);
-    float us = rnd(seed);
-    int sel = S.nLights - 1;
-    for (int i = 0; i < S.nLights; ++i) {
-        if (STATS) C.lightScans++;
-        if (us < S.cdf[i]) { sel = i; break; }
-    }
-    if (STATS) C.lightReads++;
-    const float4 *L = S.lights + 5 * (size_t)sel;
-    float4 L0 = L[0], L1 = L[1], L2 = L[2], L3 = L[3], L4 = L[4];
-    float pickPdf = L3.w / S.totalLightPower;
-    if (pickPdf <= 0.0f) return mk(0, 0, 0);
-    float r1 = rnd(seed);
-    float r2 = rnd(seed);
-    if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
-    v3 lv0 = xyz(L0), lv1 = xyz(L1), lv2 = xyz(L2);
-    v3 lp = (lv0 + (lv1 - lv0) * r1) + (lv2 - lv0) * r2;
-    v3 tl = lp - hp;
-    float d2 = dot(tl, tl);
-    float d = sqrtf(d2);
-    v3 sd = tl / d;
-    float cosHit = dot(n, sd);
-    if (cosHit <= 0.0f) return mk(0, 0, 0);
-    float cosLight = fabsf(dot(xyz(L4), sd));
-    if (cosLight <= 0.0f) return mk(0, 0, 0);
-    float area = L0.w;
-    float d2s = fmaxf(d2, area);
-    v3 so = hp + n * 2e-3f;
-    if (!unoccluded<STATS>(S, so, sd, d - 4e-3f, C)) return mk(0, 0, 0);
-    float pdfSA = d2s / (area * cosLight);
-    float pdf = pickPdf * pdfSA;
-    if (pdf <= 0.0f) return mk(0, 0, 0);
-    v3 em = xyz(L3);
-    v3 c = mk((((albedo.x / RC_PI_F) * em.x) * cosHit) / pdf, (((albedo.y / RC_PI_F) * em.y) * cosHit) / pdf,
-              (((albedo.z / RC_PI_F) * em.z) * cosHit) / pdf);
-    float emLum = (0.299f * em.x + 0.587f * em.y) + 0.114f * em.z;
-    float cl = emLum * A.neeClamp;
-    return mk(fminf(c.x, cl), fminf(c.y, cl), fminf(c.z, cl));
-}
-
-// traceRadiance (pathtrace.comp:454-585).
-template <bool STATS>
-__device__ v3 trace_radiance(const PassArgs &A, v3 ro, v3 rd, uint32_t &seed, v3 &firstAlbedo,
-                             int &firstTri, Counters &C) {
-    const SceneView &S = A.S;
-    firstAlbedo = mk(1.0f, 1.0f, 1.0f);
-    firstTri = -1;
-    v3 rad = mk(0, 0, 0), thr = mk(1.0f, 1.0f, 1.0f);
-    bool lastDiffuse = false;
-    for (int bounce = 0; bounce < A.maxBounces; ++bounce) {
-        HitInfo h = trace_closest<STATS>(S, ro, rd, C);
-        if (!h.valid) break;
-        if (STATS) C.shadeHits++;
-        const float4 *M = S.mats + 4 * (size_t)h.mat;
-        float4 m0 = M[0], m1 = M[1], m2 = M[2];
-        v3 albedo = xyz(m0);
-        float metallic = m0.w, estr = m1.w, rough = m2.x, ior = m2.y, trans = m2.z;
-        v3 emis = xyz(m1);
-        v3 n = dot(rd, h.normal) < 0.0f ? h.normal : neg(h.normal);
-        if (bounce == 0) {
-            firstAlbedo = (trans > 0.5f) ? mk(1.0f, 1.0f, 1.0f) : albedo;
-            firstTri = h.tri;
-        }
-        if (!lastDiffuse) rad = rad + (thr * emis) * estr;
-        if (bounce >= 2) {
-            float surv = fmaxf(albedo.x, fmaxf(albedo.y, albedo.z));
-            if (metallic > 0.5f || trans > 0.5f) surv = 0.95f;
-            surv = rc_clampf(surv, 0.01f, 0.95f);
-            if (rnd(seed) > surv) break;
-            thr = thr / surv;
-        }
-        if (trans > 0.5f) {
-            bool entering = dot(rd, h.normal) < 0.0f;
-            float eta = entering ? (1.0f / ior) : ior;
-            float cosI = fabsf(dot(rd, n));
-            float r0 = (1.0f - ior) / (1.0f + ior);
-            r0 = r0 * r0;
-            float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
-            if (rnd(seed) < F) {
-                rd = reflect(rd, n);
-                ro = h.pos + rd * 2e-3f;
-            } else {
-                v3 rf = refract(rd, n, eta);
-                if (dot(rf, rf) < 0.001f) rf = reflect(rd, n);
-                rd = normalize(rf);
-                ro = h.pos + rd * 2e-3f;
-            }
-            thr = thr * albedo;
-            lastDiffuse = false;
-        } else if (metallic > 0.5f) {
-            v3 wo = neg(rd);
-            float alpha = fmaxf(rough, 1e-3f);
-            v3 up = fabsf(n.y) < 0.9999f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
-            v3 T = normalize(cross(up, n));
-            v3 B = cross(n, T);
-            v3 wl = mk(dot(wo, T), dot(wo, n), dot(wo, B));
-            float u1 = rnd(seed);
-            float u2 = rnd(seed);
-            v3 hl = sample_vndf(wl, alpha, u1, u2);
-            v3 hv = normalize((T * hl.x + n * hl.y) + B * hl.z);
-            v3 mwo = neg(wo);
-            rd = reflect(mwo, hv);
-            if (dot(rd, n) <= 0.0f) rd = reflect(mwo, n);
-            float nl = fmaxf(dot(n, rd), 1e-4f);
-            float nv = fmaxf(dot(n, wo), 1e-4f);
-            float hdv = fmaxf(dot(hv, wo), 1e-4f);
-            float pw = rc_powf(1.0f - hdv, 5.0f);
-            v3 F = mk(albedo.x + (1.0f - albedo.x) * pw, albedo.y + (1.0f - albedo.y) * pw,
-                      albedo.z + (1.0f - albedo.z) * pw);
-            float g = smith_g2(nl, nv, alpha) / fmaxf(smith_g1(nv, alpha), 1e-7f);
-            thr = thr * (F * g);
-            ro = h.pos + n * 2e-3f;
-            lastDiffuse = false;
-        } else {
-            if (A.nee != 0) rad = rad + thr * direct_light<STATS>(A, h.pos, n, albedo, seed, C);
-            float u1 = rnd(seed);
-            float u2 = rnd(seed);
-            rd = sample_cosine(u1, u2, n);
-            thr = thr * albedo;
-            ro = h.pos + n * 1e-3f;
-            lastDiffuse = true;
-        }
-    }
-    return rad;
+// Interpolated shading normal and material of a hit (pathtrace.comp:166-172).
+__device__ __forceinline__ v3 hit_normal(const SceneView &S, const Hit &h, int &mat) {
+    const float4 *sh = S.triShade + 3 * (size_t)h.tri;
+    float4 a = sh[0], b = sh[1], c = sh[2];
+    float w = (1.0f - h.bu) - h.bv;
+    mat = rc_f2i_bits(a.w);
+    return normalize((xyz(a) * w + xyz(b) * h.bu) + xyz(c) * h.bv);
 }
 
 __device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const Counters &C) {
@@ -279,96 +135,255 @@ __device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const
 #pragma unroll
     for (int k = 0; k < (int)(sizeof(Counters) / 4); ++k) {
         uint32_t v = c[k];
-        // wave64 reduction by DPP-free shuffles
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(&dst[k], (unsigned long long)v);
     }
 }
 
-template <bool STATS>
-__global__ void __launch_bounds__(256) pathtrace_kernel(PassArgs A) {
-    // 16x16 tile per workgroup, 8x8 per wave.
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+template <bool STATS, bool FP64>
+__global__ void __launch_bounds__(kBlock) pathtrace_kernel(PassArgs A) {
+    __shared__ int2 stk[kShortStack][kBlock];
+    const StackRef st{stk, (int)threadIdx.x};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     Counters C;
     if (STATS) C = Counters{};
-    bool inside = px < A.cam.width && py < A.cam.height;
-    if (inside) {
-        size_t pi = (size_t)py * (size_t)A.cam.width + (size_t)px;
-        const CamArgs &cam = A.cam;
-        bool lca = (fabsf(cam.lca[0]) + fabsf(cam.lca[2])) > 1e-4f;
+    const SceneView &S = A.S;
+    const CamArgs &cam = A.cam;
+    if (px < cam.width && py < cam.height) {
+        const size_t pi = (size_t)py * (size_t)cam.width + (size_t)px;
+        const bool lca = (fabsf(cam.lca[0]) + fabsf(cam.lca[2])) > 1e-4f;
         float4 acc = make_float4(0, 0, 0, 0);
         double4 acc64 = make_double4(0, 0, 0, 0);
         float var = 0.0f;
         if (A.frame0 > 0) {
-            acc = A.accum[pi];
+            if (FP64) acc64 = A.accum64[pi];
+            else acc = A.accum[pi];
             var = A.variance[pi];
-            if (A.fp64) acc64 = A.accum64[pi];
         }
-        int lastTri = -1;
-        for (int k = 0; k < A.nSamples; ++k) {
-            int frame = A.frame0 + k;
-            v3 rad, alb;
-            int tri;
-            v3 ro, rd;
-            if (lca) {
-                uint32_t sr = init_seed(px, py, frame * 3 + 0);
-                uint32_t sg = init_seed(px, py, frame * 3 + 1);
-                uint32_t sb = init_seed(px, py, frame * 3 + 2);
-                v3 aR, aG, aB;
-                int tR, tG, tB;
-                gen_ray(cam, px, py, sr, cam.lca[0], ro, rd);
-                float r = trace_radiance<STATS>(A, ro, rd, sr, aR, tR, C).x;
-                gen_ray(cam, px, py, sg, cam.lca[1], ro, rd);
-                float g = trace_radiance<STATS>(A, ro, rd, sg, aG, tG, C).y;
-                gen_ray(cam, px, py, sb, cam.lca[2], ro, rd);
-                float b = trace_radiance<STATS>(A, ro, rd, sb, aB, tB, C).z;
-                rad = mk(r, g, b);
-                alb = aG;
-                tri = tG;
-                if (STATS) C.paths += 3;
-            } else {
-                uint32_t sd = init_seed(px, py, frame);
-                gen_ray(cam, px, py, sd, 0.0f, ro, rd);
-                rad = trace_radiance<STATS>(A, ro, rd, sd, alb, tri, C);
-                if (STATS) C.paths += 1;
+        v3 ro, rd;
+        if (A.frame0 == 0) {
+            // G-buffer from the seed-99991 camera ray (pathtrace.comp:590-595, 643-646)
+            uint32_t gs = init_seed(px, py, 99991);
+            gen_ray(cam, px, py, gs, 0.0f, ro, rd);
+            Hit h = closest_hit<STATS>(S, ro, rd, st, C);
+            float g0 = 0.0f, g1 = 1.0f, g2 = 0.0f, g3 = 1e4f;
+            if (h.tri >= 0) {
+                int m;
+                v3 nn = hit_normal(S, h, m);
+                v3 n = dot(rd, nn) < 0.0f ? nn : neg(nn);
+                g0 = n.x; g1 = n.y; g2 = n.z; g3 = h.t;
             }
-            if (isnan(rad.x) || isnan(rad.y) || isnan(rad.z) || isinf(rad.x) || isinf(rad.y) || isinf(rad.z))
-                rad = mk(0, 0, 0);
-            float luma = (rad.x * 0.2126f + rad.y * 0.7152f) + rad.z * 0.0722f;
-            float l2 = luma * luma;
-            lastTri = tri;
-            if (frame == 0) {
-                acc = make_float4(rad.x, rad.y, rad.z, 1.0f);
-                acc64 = make_double4(rad.x, rad.y, rad.z, 1.0);
-                var = l2;
-                uint32_t gs = init_seed(px, py, 99991);
-                gen_ray(cam, px, py, gs, 0.0f, ro, rd);
-                HitInfo h = trace_closest<STATS>(A.S, ro, rd, C);
-                float g0, g1, g2, g3;
-                if (!h.valid) { g0 = 0.0f; g1 = 1.0f; g2 = 0.0f; g3 = 1e4f; }
-                else {
-                    v3 n = dot(rd, h.normal) < 0.0f ? h.normal : neg(h.normal);
-                    g0 = n.x; g1 = n.y; g2 = n.z; g3 = h.t;
+            A.gbuf[pi] = make_ushort4(rc_f32_to_f16(g0), rc_f32_to_f16(g1), rc_f32_to_f16(g2), rc_f32_to_f16(g3));
+            A.gbufId[pi] = h.tri;
+        }
+
+        // path state
+        int k = 0, ch = 0, bounce = 0;
+        uint32_t seed = 0;
+        v3 thr = mk(1, 1, 1), rad = mk(0, 0, 0);
+        float sR = 0.0f, sG = 0.0f, sB = 0.0f;  // radiance of the current sample
+        bool lastDiffuse = false, fresh = true;
+        int firstTri = -1, lastTri = -1;
+        while (true) {
+            if (fresh) {
+                if (k >= A.nSamples) break;
+                const int frame = A.frame0 + k;
+                float fOff = 0.0f;
+                if (lca) {
+                    seed = init_seed(px, py, frame * 3 + ch);
+                    fOff = cam.lca[ch];
+                } else {
+                    seed = init_seed(px, py, frame);
                 }
-                ushort4 gb;
-                gb.x = rc_f32_to_f16(g0); gb.y = rc_f32_to_f16(g1);
-                gb.z = rc_f32_to_f16(g2); gb.w = rc_f32_to_f16(g3);
-                A.gbuf[pi] = gb;
-                A.gbufId[pi] = h.valid ? h.tri : -1;
-                ushort4 ga;
-                ga.x = rc_f32_to_f16(alb.x); ga.y = rc_f32_to_f16(alb.y);
-                ga.z = rc_f32_to_f16(alb.z); ga.w = rc_f32_to_f16(1.0f);
-                A.galbedo[pi] = ga;
+                gen_ray(cam, px, py, seed, fOff, ro, rd);
+                thr = mk(1.0f, 1.0f, 1.0f);
+                rad = mk(0.0f, 0.0f, 0.0f);
+                bounce = 0;
+                lastDiffuse = false;
+                fresh = false;
+                if (STATS) C.paths++;
+            }
+            bool end = false;
+            Hit h = closest_hit<STATS>(S, ro, rd, st, C);
+            if (h.tri < 0) {
+                end = true;  // black sky (pathtrace.comp:470-473)
+                if (bounce == 0 && (!lca || ch == 1) && A.frame0 + k == 0) {
+                    const uint16_t one = rc_f32_to_f16(1.0f);  // miss: albedo stays (1,1,1) (:455)
+                    A.galbedo[pi] = make_ushort4(one, one, one, one);
+                }
             } else {
-                acc.x += rad.x; acc.y += rad.y; acc.z += rad.z; acc.w += 1.0f;
-                acc64.x += (double)rad.x; acc64.y += (double)rad.y; acc64.z += (double)rad.z;
-                acc64.w += 1.0;
-                var += l2;
+                if (STATS) C.shadeHits++;
+                int mi;
+                v3 hn = hit_normal(S, h, mi);
+                v3 hp = ro + rd * h.t;
+                const float4 *M = S.mats + 4 * (size_t)mi;
+                float4 m0 = M[0], m1 = M[1], m2 = M[2];
+                v3 albedo = xyz(m0);
+                float metallic = m0.w, estr = m1.w, rough = m2.x, ior = m2.y, trans = m2.z;
+                v3 n = dot(rd, hn) < 0.0f ? hn : neg(hn);
+                if (bounce == 0 && (!lca || ch == 1)) {
+                    // first-hit albedo (pathtrace.comp:481-483); the G channel's under LCA
+                    firstTri = h.tri;
+                    if (A.frame0 + k == 0) {
+                        v3 fa = (trans > 0.5f) ? mk(1.0f, 1.0f, 1.0f) : albedo;
+                        A.galbedo[pi] = make_ushort4(rc_f32_to_f16(fa.x), rc_f32_to_f16(fa.y),
+                                                     rc_f32_to_f16(fa.z), rc_f32_to_f16(1.0f));
+                    }
+                }
+                if (!lastDiffuse) rad = rad + (thr * xyz(m1)) * estr;
+                if (bounce >= 2) {
+                    float surv = fmaxf(albedo.x, fmaxf(albedo.y, albedo.z));
+                    if (metallic > 0.5f || trans > 0.5f) surv = 0.95f;
+                    surv = rc_clampf(surv, 0.01f, 0.95f);
+                    if (rnd(seed) > surv) end = true;
+                    else thr = thr / surv;
+                }
+                // The direction sampled at the last bounce is never traced and its
+                // random numbers are never reused (each path re-seeds): skipped.
+                const bool last = bounce + 1 >= A.maxBounces;
+                if (!end) {
+                    if (trans > 0.5f) {
+                        if (!last) {
+                            bool entering = dot(rd, hn) < 0.0f;
+                            float eta = entering ? (1.0f / ior) : ior;
+                            float cosI = fabsf(dot(rd, n));
+                            float r0 = (1.0f - ior) / (1.0f + ior);
+                            r0 = r0 * r0;
+                            float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
+                            if (rnd(seed) < F) {
+                                rd = reflect(rd, n);
+                            } else {
+                                v3 rf = refract(rd, n, eta);
+                                if (dot(rf, rf) < 0.001f) rf = reflect(rd, n);
+                                rd = normalize(rf);
+                            }
+                            ro = hp + rd * 2e-3f;
+                        }
+                        thr = thr * albedo;
+                        lastDiffuse = false;
+                    } else if (metallic > 0.5f) {
+                        if (!last) {
+                            v3 wo = neg(rd);
+                            float alpha = fmaxf(rough, 1e-3f);
+                            v3 up = fabsf(n.y) < 0.9999f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+                            v3 T = normalize(cross(up, n));
+                            v3 B = cross(n, T);
+                            v3 wl = mk(dot(wo, T), dot(wo, n), dot(wo, B));
+                            float u1 = rnd(seed);
+                            float u2 = rnd(seed);
+                            v3 hl = sample_vndf(wl, alpha, u1, u2);
+                            v3 hv = normalize((T * hl.x + n * hl.y) + B * hl.z);
+                            rd = reflect(rd, hv);
+                            if (dot(rd, n) <= 0.0f) rd = reflect(neg(wo), n);
+                            float nl = fmaxf(dot(n, rd), 1e-4f);
+                            float nv = fmaxf(dot(n, wo), 1e-4f);
+                            float hdv = fmaxf(dot(hv, wo), 1e-4f);
+                            float pw = rc_powf(1.0f - hdv, 5.0f);
+                            v3 F = mk(albedo.x + (1.0f - albedo.x) * pw, albedo.y + (1.0f - albedo.y) * pw,
+                                      albedo.z + (1.0f - albedo.z) * pw);
+                            float g = smith_g2(nl, nv, alpha) / fmaxf(smith_g1(nv, alpha), 1e-7f);
+                            thr = thr * (F * g);
+                            ro = hp + n * 2e-3f;
+                        }
+                        lastDiffuse = false;
+                    } else {
+                        if (A.nee != 0) {
+                            // sampleDirectLight (pathtrace.comp:341-395)
+                            v3 nee = mk(0.0f, 0.0f, 0.0f);
+                            if (S.nLights > 0 && S.totalLightPower > 0.0f) {
+                                float us = rnd(seed);
+                                int sel = S.nLights - 1;
+                                for (int i = 0; i < S.nLights; ++i) {
+                                    if (STATS) C.lightScans++;
+                                    if (us < S.cdf[i]) { sel = i; break; }
+                                }
+                                if (STATS) C.lightReads++;
+                                const float4 *L = S.lights + 5 * (size_t)sel;
+                                float4 L0 = L[0], L3 = L[3];
+                                float pickPdf = L3.w / S.totalLightPower;
+                                if (pickPdf > 0.0f) {
+                                    float4 L1 = L[1], L2 = L[2], L4 = L[4];
+                                    float r1 = rnd(seed);
+                                    float r2 = rnd(seed);
+                                    if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+                                    v3 lv0 = xyz(L0);
+                                    v3 lp = (lv0 + (xyz(L1) - lv0) * r1) + (xyz(L2) - lv0) * r2;
+                                    v3 tl = lp - hp;
+                                    float d2 = dot(tl, tl);
+                                    float d = sqrtf(d2);
+                                    v3 sd = tl / d;
+                                    float cosHit = dot(n, sd);
+                                    float cosLight = fabsf(dot(xyz(L4), sd));
+                                    if (cosHit > 0.0f && cosLight > 0.0f &&
+                                        unoccluded<STATS>(S, hp + n * 2e-3f, sd, d - 4e-3f, st, C)) {
+                                        float area = L0.w;
+                                        float pdf = pickPdf * (fmaxf(d2, area) / (area * cosLight));
+                                        if (pdf > 0.0f) {
+                                            v3 em = xyz(L3);
+                                            float cl = ((0.299f * em.x + 0.587f * em.y) + 0.114f * em.z) * A.neeClamp;
+                                            nee = mk(fminf((((albedo.x / RC_PI_F) * em.x) * cosHit) / pdf, cl),
+                                                     fminf((((albedo.y / RC_PI_F) * em.y) * cosHit) / pdf, cl),
+                                                     fminf((((albedo.z / RC_PI_F) * em.z) * cosHit) / pdf, cl));
+                                        }
+                                    }
+                                }
+                            }
+                            rad = rad + thr * nee;
+                        }
+                        if (!last) {
+                            float u1 = rnd(seed);
+                            float u2 = rnd(seed);
+                            rd = sample_cosine(u1, u2, n);
+                            ro = hp + n * 1e-3f;
+                        }
+                        thr = thr * albedo;
+                        lastDiffuse = true;
+                    }
+                }
+                ++bounce;
+                if (last) end = true;
+            }
+            if (end) {
+                if (!lca) {
+                    sR = rad.x; sG = rad.y; sB = rad.z;
+                } else if (ch == 0) {
+                    sR = rad.x;
+                } else if (ch == 1) {
+                    sG = rad.y;
+                } else {
+                    sB = rad.z;
+                }
+                fresh = true;
+                if (lca && ch < 2) {
+                    ++ch;
+                } else {
+                    // sample complete (pathtrace.comp:633-654)
+                    ch = 0;
+                    if (isnan(sR) || isnan(sG) || isnan(sB) || isinf(sR) || isinf(sG) || isinf(sB))
+                        sR = sG = sB = 0.0f;
+                    float luma = (sR * 0.2126f + sG * 0.7152f) + sB * 0.0722f;
+                    float l2 = luma * luma;
+                    if (A.frame0 + k == 0) {
+                        acc = make_float4(sR, sG, sB, 1.0f);
+                        acc64 = make_double4(sR, sG, sB, 1.0);
+                        var = l2;
+                    } else {
+                        if (FP64) {
+                            acc64.x += (double)sR; acc64.y += (double)sG; acc64.z += (double)sB; acc64.w += 1.0;
+                        } else {
+                            acc.x += sR; acc.y += sG; acc.z += sB; acc.w += 1.0f;
+                        }
+                        var += l2;
+                    }
+                    lastTri = firstTri;
+                    firstTri = -1;
+                    ++k;
+                }
             }
         }
-        if (A.fp64) {
+        if (FP64) {
             A.accum64[pi] = acc64;
             acc = make_float4((float)acc64.x, (float)acc64.y, (float)acc64.z, (float)acc64.w);
         }
@@ -399,7 +414,6 @@ hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_param
     A.maxBounces = p.maxBounces;
     A.frame0 = ctx->totalSamples;
     A.nSamples = nSamples;
-    A.fp64 = ctx->accumMode == RC_ACCUM_FP64 ? 1 : 0;
     A.accum = ctx->img.accum;
     A.accum64 = ctx->img.accum64;
     A.gbuf = ctx->img.gbuf;
@@ -409,10 +423,12 @@ hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_param
     A.gbufId = ctx->img.gbufId;
     A.counters = ctx->counters;
     dim3 grid((cam.width + 15) / 16, (cam.height + 15) / 16);
-    if (ctx->countersEnabled && ctx->counters)
-        hipLaunchKernelGGL(pathtrace_kernel<true>, grid, dim3(256), 0, ctx->stream, A);
-    else
-        hipLaunchKernelGGL(pathtrace_kernel<false>, grid, dim3(256), 0, ctx->stream, A);
+    const bool stats = ctx->countersEnabled && ctx->counters;
+    const bool fp64 = ctx->accumMode == RC_ACCUM_FP64;
+    if (stats && fp64) hipLaunchKernelGGL((pathtrace_kernel<true, true>), grid, dim3(kBlock), 0, ctx->stream, A);
+    else if (stats) hipLaunchKernelGGL((pathtrace_kernel<true, false>), grid, dim3(kBlock), 0, ctx->stream, A);
+    else if (fp64) hipLaunchKernelGGL((pathtrace_kernel<false, true>), grid, dim3(kBlock), 0, ctx->stream, A);
+    else hipLaunchKernelGGL((pathtrace_kernel<false, false>), grid, dim3(kBlock), 0, ctx->stream, A);
     return hipGetLastError();
 }
 

@@ -112,7 +112,8 @@ class Renderer:
 
     _DT = {N.RC_BUF_ACCUM: (np.float32, 4), N.RC_BUF_GBUFFER: (np.uint16, 4), N.RC_BUF_GALBEDO: (np.uint16, 4),
            N.RC_BUF_VARIANCE: (np.float32, 1), N.RC_BUF_PRIMARY_ID: (np.int32, 1),
-           N.RC_BUF_GBUFFER_ID: (np.int32, 1), N.RC_BUF_ACCUM64: (np.float64, 4)}
+           N.RC_BUF_GBUFFER_ID: (np.int32, 1), N.RC_BUF_ACCUM64: (np.float64, 4),
+           N.RC_BUF_POST_HDR: (np.float32, 4), N.RC_BUF_DISPLAY: (np.uint8, 4)}
 
     def read(self, which: int) -> np.ndarray:
         dt, ch = self._DT[which]
