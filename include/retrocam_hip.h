@@ -211,6 +211,35 @@ int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes);
 /* Size in bytes of a readable buffer at the current resolution. */
 size_t rc_buffer_bytes(const rc_ctx *ctx, int32_t which);
 
+/* ---- SPPM caustics ------------------------------------------------------- */
+/* SPPMManager.reset (src/main/java/com/retrocam/scene/SPPMManager.java:180-183):
+ * radius := initRadius, iteration := 0. */
+int rc_sppm_reset(rc_ctx *ctx, float initRadius);
+/* 0 (default): the radius is a float updated as the reference does; 1: the
+ * schedule runs in FP64 and the kernels use its rounding to float. */
+int rc_sppm_set_precision(rc_ctx *ctx, int32_t fp64Radius);
+/* SPPMManager.tracePhotons (SPPMManager.java:76-101): clears the hash heads to -1
+ * and the counter to 0, then emits photonCount photons with seed
+ * iteration*7919 + 6271 at the current radius and inserts caustic photons into
+ * the hash (hashSize cells, a power of two). photonCount <= 32,000,000,
+ * hashSize <= 2^26. A scene without lights stores nothing. */
+int rc_sppm_trace(rc_ctx *ctx, int32_t photonCount, int32_t hashSize);
+/* SPPMManager.gatherRadiance (SPPMManager.java:118-160): pinhole ray per pixel
+ * from cam's position/basis/fovTanHalf (lens ignored, aspect = width/height),
+ * through specular surfaces to the first diffuse hit, 3x3x3-cell density
+ * estimate, accumulation += (L, 0). cam's resolution must equal the
+ * accumulation's. */
+int rc_sppm_gather(rc_ctx *ctx, const rc_camera *cam);
+/* SPPMManager.updateRadius (SPPMManager.java:170-173), then iteration += 1. */
+int rc_sppm_update_radius(rc_ctx *ctx, float alpha);
+/* Current radius (float the kernels use), iteration and the FP64 schedule value. */
+int rc_sppm_get_state(const rc_ctx *ctx, float *radius, int32_t *iteration, double *radius64);
+/* Copies the photon map of the last trace: photons (photonCount x 12 floats:
+ * position, pad, power, pad, direction, pad), heads (hashSize), next
+ * (photonCount) and the raw counter; any pointer may be NULL. Slots at or above
+ * the counter are not written by the trace. */
+int rc_sppm_read(rc_ctx *ctx, float *photons, int32_t *heads, int32_t *next, uint32_t *counter);
+
 /* ---- post chain ---------------------------------------------------------- */
 /* Fills p with the RenderSettings defaults (RenderSettings.java:26-191):
  * a-trous off (4 iterations), every VHS pass off. */

@@ -95,6 +95,26 @@ int32_t rco_post_pass(int32_t pass, const float *src, float *dst, int32_t W, int
                       const rc_temporal *t, const uint16_t *gbuf, const uint16_t *galbedo, const float *variance,
                       int32_t totalSamples);
 
+/* ---- SPPM (rc_oracle_sppm.c) ---- */
+/* Photon seed initSeed(idx) (photon_trace.comp:68-72). */
+uint32_t rco_photon_seed(uint32_t idx, int32_t seed);
+/* cellHash of integer cell coordinates (photon_trace.comp:140-143). */
+uint32_t rco_cell_hash(int32_t cx, int32_t cy, int32_t cz, int32_t hashSize);
+/* Unordered closest hit of the photon shaders; triangle index or -1. */
+int32_t rco_walk_closest(const rco_scene *s, const float *ro, const float *rd, float *tOut);
+/* PhotonMap.clear + photon_trace.comp over photonCount photons; photons is
+ * photonCount x 12 floats, heads hashSize ints, next photonCount ints. Photons are
+ * inserted in photon-index order. *storedOut = number of stored photons. */
+int32_t rco_photon_trace(const rco_scene *s, int32_t photonCount, float radius, int32_t seed, int32_t hashSize,
+                         float *photons, int32_t *heads, int32_t *next, int32_t *storedOut);
+/* photon_gather.comp for every pixel of cam (pinhole, aspect = width/height):
+ * accum (float4 per pixel) += (L, 0); accum64 (optional) += L in double. */
+int32_t rco_photon_gather(const rco_scene *s, const rc_camera *cam, float radius, int32_t hashSize,
+                          const float *photons, const int32_t *heads, const int32_t *next, float *accum,
+                          double *accum64);
+/* SPPMManager.updateRadius with Java float/double semantics. */
+float rco_sppm_next_radius(float r, int32_t iteration, float alpha);
+
 /* VHS integer hashes: kind 0 pcgHash (p07_ccd_noise.frag:50-54), 1 one-round
  * uHash (p12_dropout.frag:32-37), 2 two-round uHash (p13_headswitch.frag:40-45),
  * 3 / 4 the float bits of fHash built on 1 / 2. */

@@ -109,6 +109,13 @@ HIP_SYMBOLS = {
     "rc_set_accum_mode": (C.c_int, [_VP, C.c_int32]),
     "rc_read": (C.c_int, [_VP, C.c_int32, _VP, C.c_size_t]),
     "rc_buffer_bytes": (C.c_size_t, [_VP, C.c_int32]),
+    "rc_sppm_reset": (C.c_int, [_VP, C.c_float]),
+    "rc_sppm_set_precision": (C.c_int, [_VP, C.c_int32]),
+    "rc_sppm_trace": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    "rc_sppm_gather": (C.c_int, [_VP, C.POINTER(RcCamera)]),
+    "rc_sppm_update_radius": (C.c_int, [_VP, C.c_float]),
+    "rc_sppm_get_state": (C.c_int, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "rc_sppm_read": (C.c_int, [_VP, _FP, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]),
     "rc_post_defaults": (None, [C.POINTER(RcPostParams)]),
     "rc_post": (C.c_int, [_VP, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
     "rc_post_image": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),

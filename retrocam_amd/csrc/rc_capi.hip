@@ -218,6 +218,7 @@ int rc_destroy(rc_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx->scene);
     free_images(ctx->img);
+    free_sppm(ctx->sppm);
     dfree(ctx->counters);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

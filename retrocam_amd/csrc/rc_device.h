@@ -216,6 +216,71 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Sta
     return h;
 }
 
+// Box of stack entry e = (parent << 1 | side), or the root for e < 0; returns the
+// child ref that box belongs to.
+__device__ __forceinline__ int entry_box(const SceneView &S, int e, float &mnx, float &mny, float &mnz,
+                                         float &mxx, float &mxy, float &mxz) {
+    if (e < 0) {
+        mnx = S.rootMin[0]; mny = S.rootMin[1]; mnz = S.rootMin[2];
+        mxx = S.rootMax[0]; mxy = S.rootMax[1]; mxz = S.rootMax[2];
+        return S.rootRef;
+    }
+    const float4 *n = S.inner + 4 * (size_t)(e >> 1);
+    int4 r = ((const int4 *)n)[3];
+    if ((e & 1) == 0) {
+        float4 a = n[0], b = n[1];
+        mnx = a.x; mny = a.y; mnz = a.z; mxx = a.w; mxy = b.x; mxz = b.y;
+        return r.x;
+    }
+    float4 b = n[1], c = n[2];
+    mnx = b.z; mny = b.w; mnz = c.x; mxx = c.y; mxy = c.z; mxz = c.w;
+    return r.y;
+}
+
+// Closest hit of the photon shaders (photon_trace.comp:101-124,
+// photon_gather.comp:85-118): unordered DFS, pop tests the node's own box and
+// culls at aabbT > rec.t, pushes right then left, 62-entry cap.
+__device__ __forceinline__ Hit closest_hit_unordered(const SceneView &S, v3 ro, v3 rd, StackRef st) {
+    Hit h;
+    h.t = 1e30f; h.tri = -1; h.bu = 0.0f; h.bv = 0.0f;
+    v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int spill[64 - kShortStack];
+    int top = 0;
+    int e = -1;
+    bool have = true;
+    while (true) {
+        if (!have) {
+            if (top == 0) break;
+            --top;
+            e = top < kShortStack ? st.lds[top][st.lane].x : spill[top - kShortStack];
+        }
+        have = false;
+        float mnx, mny, mnz, mxx, mxy, mxz;
+        int ref = entry_box(S, e, mnx, mny, mnz, mxx, mxy, mxz);
+        float at;
+        if (!aabb(ro, inv, mnx, mny, mnz, mxx, mxy, mxz, at)) continue;
+        if (at > h.t) continue;
+        if (ref < 0) {
+            int start, count;
+            leaf_range(S, ref, start, count);
+            for (int i = start; i < start + count; ++i) {
+                float t, bu, bv;
+                if (tri_test(S, ro, rd, i, t, bu, bv) && t < h.t) {
+                    h.t = t; h.bu = bu; h.bv = bv; h.tri = i;
+                }
+            }
+        } else if (top < 62) {
+            int right = (ref << 1) | 1;
+            if (top < kShortStack) st.lds[top][st.lane].x = right;
+            else spill[top - kShortStack] = right;
+            ++top;
+            e = ref << 1;
+            have = true;
+        }
+    }
+    return h;
+}
+
 // shadowRay (pathtrace.comp:211-242): unordered DFS with the 30-entry cap. Both
 // children are pushed untested; the pop tests the node's own box. Boxes of inner
 // nodes live in their parent, so each stack entry names (parent << 1 | side);

@@ -62,6 +62,21 @@ inline SceneView scene_view(const DeviceScene &D) {
     return S;
 }
 
+// SPPM state (SPPMManager + PhotonMap, src/main/java/com/retrocam/scene/).
+struct Sppm {
+    float radius = 0.05f;      // searchRadius (RenderSettings.sppmInitRadius default)
+    double radius64 = 0.05;    // FP64 radius schedule (rc_sppm_set_precision)
+    int fp64Radius = 0;
+    int iteration = 0;
+    int photonCount = 0;       // photons of the last trace
+    int hashSize = 0;          // hash cells of the last trace
+    int capPhotons = 0, capHash = 0;
+    float4 *photons = nullptr; // 3 float4 per photon: position, power, direction
+    int *heads = nullptr;
+    int *next = nullptr;
+    unsigned int *counter = nullptr;
+};
+
 }  // namespace rc
 
 struct rc_ctx {
@@ -72,6 +87,7 @@ struct rc_ctx {
     rc::Images img;
     int32_t totalSamples = 0;
     int32_t accumMode = RC_ACCUM_FP32;
+    rc::Sppm sppm;
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
     int countersEnabled = 0;
 };
@@ -83,6 +99,9 @@ int hip_fail(rc_ctx *ctx, hipError_t e, const char *what);
 
 // Launches nSamples progressive passes starting at frame ctx->totalSamples.
 hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
+
+// Releases the SPPM buffers (rc_sppm.hip).
+void free_sppm(Sppm &s);
 
 // One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
 // with the given step, 200 = display encode into rgba8.

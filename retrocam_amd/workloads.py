@@ -104,6 +104,42 @@ def full_frame(seed: int = 1, n_tris: int = 250_000, width: int = 854, height: i
     return Workload("full_frame", scene, cam, PtParams(1, clamp, 6), spp, {"seed": seed, "n_tris": n_tris})
 
 
+def sppm_caustics(seed: int = 1, width: int = 854, height: int = 480, spp: int = 64, photons: int = 2_000_000,
+                  hash_size: int = 1 << 22, init_radius: float = 0.02, alpha: float = 0.7,
+                  sphere_res=(32, 64)) -> Workload:
+    """The cornell_small scene plus three ior-1.5 glass spheres (32 stacks x 64
+    slices = 3968 triangles each) under a small 1x1 quad light."""
+    w = cornell_small(seed, width=width, height=height, spp=spp)
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    # rebuild the Cornell scene (same materials and geometry), then add the spheres
+    alb = rng.uniform(0.1, 0.8, size=(8, 3)).astype(np.float32)
+    m = [b.add_material(Material.diffuse(*alb[i])) for i in range(8)]
+    light = b.add_material(Material.emissive(1.0, 1.0, 1.0, 15.0))
+    x0, x1, y0, y1, z0, z1 = -5.0, 5.0, 0.0, 10.0, -5.0, 5.0
+    b.add_quad((x0, y0, z0), (x0, y0, z1), (x1, y0, z1), (x1, y0, z0), m[0])
+    b.add_quad((x0, y1, z0), (x1, y1, z0), (x1, y1, z1), (x0, y1, z1), m[1])
+    b.add_quad((x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0), m[2])
+    b.add_quad((x0, y0, z0), (x0, y1, z0), (x0, y1, z1), (x0, y0, z1), m[3])
+    b.add_quad((x1, y0, z0), (x1, y0, z1), (x1, y1, z1), (x1, y1, z0), m[4])
+    b.add_box((-1.5, 1.5, -1.0), (1.0, 1.5, 1.0), m[5])
+    b.add_box((1.5, 1.0, 0.5), (1.0, 1.0, 1.0), m[6])
+    ly = 9.99
+    b.add_quad((-1.5, ly, -1.5), (1.5, ly, -1.5), (1.5, ly, 1.5), (-1.5, ly, 1.5), light)
+    b.add_sphere((0.3, 1.2, -3.4), 1.2, m[7], stacks=16, slices=32)
+    glass = b.add_material(Material.glass(1.5))
+    small = b.add_material(Material.emissive(1.0, 0.9, 0.7, 40.0))
+    st, sl = sphere_res
+    for cx, cz, r in ((-2.5, 2.0, 0.7), (0.0, 2.6, 0.8), (2.6, 2.2, 0.6)):
+        b.add_sphere((cx, r + 0.6, cz), r, glass, stacks=st, slices=sl)
+    sy = 6.0
+    b.add_quad((-0.5, sy, 1.9), (0.5, sy, 1.9), (0.5, sy, 2.9), (-0.5, sy, 2.9), small)
+    scene = b.build()
+    return Workload("sppm_caustics", scene, w.camera, w.params, spp,
+                    {"seed": seed, "photons": photons, "hash_size": hash_size, "init_radius": init_radius,
+                     "alpha": alpha})
+
+
 def reference_default(width: int = 854, height: int = 480, spp: int = 16) -> Workload:
     """SceneEditor.createDefault (SceneEditor.java:111-134), default orbit camera and lens."""
     b = SceneBuilder()
