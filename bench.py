@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=128)
     ap.add_argument("--cpu-spp", type=int, default=4, help="samples timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--pt-mode", choices=["wavefront", "megakernel"], default="megakernel")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -126,6 +127,8 @@ def main():
     ctx = RenderContext(0)
     SceneUploader(ctx).upload(w.scene)
     r = Renderer(ctx)
+    from retrocam_amd import _native as N
+    r.set_pt_mode(N.RC_PT_WAVEFRONT if args.pt_mode == "wavefront" else N.RC_PT_MEGAKERNEL)
 
     def frame():
         r.reset()
@@ -181,7 +184,8 @@ def main():
         "data": "synthetic (seeded 250k-triangle scene, generated in-process)",
         "config": {"workload": "full_frame", "width": w.camera.width, "height": w.camera.height,
                    "spp": w.spp, "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
-                   "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU"},
+                   "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU",
+                   "pt_mode": args.pt_mode},
         "mrays_per_s": round(rays * fps / 1e6, 2),
         "rays_per_frame": rays,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -164,6 +164,12 @@ enum {
     RC_ACCUM_FP64 = 1       /* FP64 running sum; RC_BUF_ACCUM returns it rounded   */
 };
 
+/* Path-tracer kernel organisation (rc_set_pt_mode); both give identical results. */
+enum {
+    RC_PT_MEGAKERNEL = 0,   /* one kernel, one lane per pixel for all samples     */
+    RC_PT_WAVEFRONT = 1     /* extend / shadow / shade kernels over ray queues    */
+};
+
 typedef struct rc_ctx rc_ctx;
 
 /* ---- context ------------------------------------------------------------ */
@@ -203,6 +209,9 @@ int rc_render_passes(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *para
 int rc_reset(rc_ctx *ctx);
 /* Renderer.getTotalSamples (Renderer.java:138). */
 int32_t rc_total_samples(const rc_ctx *ctx);
+/* RC_PT_MEGAKERNEL (default) or RC_PT_WAVEFRONT for later passes; both give
+   bit-identical images. */
+int rc_set_pt_mode(rc_ctx *ctx, int32_t mode);
 /* RC_ACCUM_FP32 (default) or RC_ACCUM_FP64; resets the accumulation. */
 int rc_set_accum_mode(rc_ctx *ctx, int32_t mode);
 /* Copies a device buffer (RC_BUF_*) of the current resolution to host memory;

@@ -36,6 +36,9 @@ RC_BUF_DISPLAY = 8
 RC_ACCUM_FP32 = 0
 RC_ACCUM_FP64 = 1
 
+RC_PT_MEGAKERNEL = 0
+RC_PT_WAVEFRONT = 1
+
 F3 = C.c_float * 3
 
 
@@ -107,6 +110,7 @@ HIP_SYMBOLS = {
     "rc_reset": (C.c_int, [_VP]),
     "rc_total_samples": (C.c_int32, [_VP]),
     "rc_set_accum_mode": (C.c_int, [_VP, C.c_int32]),
+    "rc_set_pt_mode": (C.c_int, [_VP, C.c_int32]),
     "rc_read": (C.c_int, [_VP, C.c_int32, _VP, C.c_size_t]),
     "rc_buffer_bytes": (C.c_size_t, [_VP, C.c_int32]),
     "rc_sppm_reset": (C.c_int, [_VP, C.c_float]),

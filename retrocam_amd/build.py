@@ -32,7 +32,8 @@ HIP_FLAGS = [
     f"-I{INC}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["rc_capi.hip", "rc_pathtrace.hip", "rc_debug.hip", "rc_post.hip", "rc_sppm.hip"]
+HIP_SOURCES = ["rc_capi.hip", "rc_pathtrace.hip", "rc_wavefront.hip", "rc_debug.hip", "rc_post.hip",
+               "rc_sppm.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -219,6 +219,7 @@ int rc_destroy(rc_ctx *ctx) {
     free_scene(ctx->scene);
     free_images(ctx->img);
     free_sppm(ctx->sppm);
+    free_wavefront(ctx->wf);
     dfree(ctx->counters);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -374,7 +375,8 @@ static int render_common(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *
         return set_error(ctx, RC_ERR_LIMIT, "sample count overflow");
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_images(ctx, cam->width, cam->height))) return rc;
-    hipError_t e = launch_pathtrace(ctx, *cam, *p, n);
+    hipError_t e = ctx->ptMode == RC_PT_WAVEFRONT ? launch_wavefront(ctx, *cam, *p, n)
+                                                  : launch_pathtrace(ctx, *cam, *p, n);
     if (e != hipSuccess) return hip_fail(ctx, e, "pathtrace launch");
     ctx->totalSamples += n;
     return RC_OK;
@@ -424,6 +426,14 @@ int rc_debug_counters(rc_ctx *ctx, int32_t enable, uint64_t *out, int32_t n) {
 }
 
 int32_t rc_total_samples(const rc_ctx *ctx) { return ctx ? ctx->totalSamples : -1; }
+
+int rc_set_pt_mode(rc_ctx *ctx, int32_t mode) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (mode != RC_PT_MEGAKERNEL && mode != RC_PT_WAVEFRONT)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "mode must be RC_PT_MEGAKERNEL or RC_PT_WAVEFRONT");
+    ctx->ptMode = mode;
+    return RC_OK;
+}
 
 int rc_set_accum_mode(rc_ctx *ctx, int32_t mode) {
     if (!ctx) return RC_ERR_INVALID_ARG;

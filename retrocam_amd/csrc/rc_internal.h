@@ -77,6 +77,20 @@ struct Sppm {
     unsigned int *counter = nullptr;
 };
 
+// Wavefront path-tracing buffers (rc_wavefront.hip), sized in 256-slot tiles.
+struct Wavefront {
+    int slots = 0;
+    float4 *stThr = nullptr, *stRad = nullptr, *stNee = nullptr, *stSmp = nullptr;
+    int *vis = nullptr;
+    int *list[2] = {nullptr, nullptr};
+    float4 *rayO[2] = {nullptr, nullptr}, *rayD[2] = {nullptr, nullptr};
+    float4 *hits = nullptr;
+    int *shSlot = nullptr;
+    float4 *shO = nullptr, *shD = nullptr;
+    int *cnt = nullptr;
+    int *hostCnt = nullptr;  // pinned
+};
+
 }  // namespace rc
 
 struct rc_ctx {
@@ -88,6 +102,8 @@ struct rc_ctx {
     int32_t totalSamples = 0;
     int32_t accumMode = RC_ACCUM_FP32;
     rc::Sppm sppm;
+    rc::Wavefront wf;
+    int32_t ptMode = RC_PT_MEGAKERNEL;
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
     int countersEnabled = 0;
 };
@@ -102,6 +118,10 @@ hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_param
 
 // Releases the SPPM buffers (rc_sppm.hip).
 void free_sppm(Sppm &s);
+
+// Wavefront form of launch_pathtrace (rc_wavefront.hip); same results.
+hipError_t launch_wavefront(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
+void free_wavefront(Wavefront &w);
 
 // One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
 // with the given step, 200 = display encode into rgba8.

@@ -1,0 +1,246 @@
+// rc_shade.h — path-tracing shading shared by the megakernel (rc_pathtrace.hip)
+// and the wavefront kernels (rc_wavefront.hip): camera rays, BSDF sampling and
+// next-event estimation, restating pathtrace.comp:252-439 and 475-581 with the
+// reference's operand order (bit-identical to the CPU oracle).
+#pragma once
+
+#include "rc_device.h"
+
+namespace rc {
+
+struct CamArgs {
+    float pos[3], right[3], up[3], fwd[3];
+    float focalDist, lensRadius, tanHalf, saStrength;
+    float lca[3];
+    int blades, width, height;
+};
+
+__device__ __forceinline__ v3 sample_vndf(v3 wo, float alpha, float ux, float uy) {
+    v3 v = normalize(mk(wo.x * alpha, wo.y, wo.z * alpha));
+    v3 t1 = (v.y < 0.9999f) ? normalize(cross(v, mk(0.0f, 1.0f, 0.0f))) : mk(1.0f, 0.0f, 0.0f);
+    v3 t2 = cross(t1, v);
+    float a = 1.0f / (1.0f + v.y);
+    float r = sqrtf(ux);
+    float phi = (uy < a) ? (uy / a) * RC_PI_F : RC_PI_F + ((uy - a) / (1.0f - a)) * RC_PI_F;
+    float p1 = r * rc_cosf(phi);
+    float p2 = (r * rc_sinf(phi)) * ((uy < a) ? 1.0f : v.y);
+    float sz = sqrtf(fmaxf(0.0f, (1.0f - p1 * p1) - p2 * p2));
+    v3 n = (t1 * p1 + t2 * p2) + v * sz;
+    return normalize(mk(alpha * n.x, fmaxf(0.0f, n.y), alpha * n.z));
+}
+
+__device__ __forceinline__ float smith_g1(float nv, float alpha) {
+    float a2 = alpha * alpha;
+    float denom = nv + sqrtf(a2 + ((1.0f - a2) * nv) * nv);
+    return (2.0f * nv) / fmaxf(denom, 1e-7f);
+}
+
+__device__ __forceinline__ float smith_g2(float nl, float nv, float alpha) {
+    float a2 = alpha * alpha;
+    float da = nv * sqrtf(a2 + ((1.0f - a2) * nl) * nl);
+    float db = nl * sqrtf(a2 + ((1.0f - a2) * nv) * nv);
+    return ((2.0f * nl) * nv) / fmaxf(da + db, 1e-7f);
+}
+
+__device__ __forceinline__ v3 sample_cosine(float ux, float uy, v3 n) {
+    float r = sqrtf(ux);
+    float phi = 6.28318530718f * uy;
+    v3 l = mk(r * rc_cosf(phi), r * rc_sinf(phi), sqrtf(fmaxf(0.0f, 1.0f - ux)));
+    v3 up = fabsf(n.y) < 0.9999f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    v3 t = normalize(cross(up, n));
+    v3 b = cross(n, t);
+    return normalize((t * l.x + b * l.y) + n * l.z);
+}
+
+// Camera ray (pathtrace.comp:401-439, sampleNGon :310-321).
+__device__ __forceinline__ void gen_ray(const CamArgs &c, int px, int py, uint32_t &seed, float fOff,
+                                        v3 &ro, v3 &rd) {
+    float aspect = (float)c.width / (float)c.height;
+    float jx = rnd(seed) - 0.5f;
+    float jy = rnd(seed) - 0.5f;
+    float ux = (((float)px + 0.5f) + jx) / (float)c.width;
+    float uy = (((float)py + 0.5f) + jy) / (float)c.height;
+    float sx = ux * 2.0f - 1.0f, sy = uy * 2.0f - 1.0f;
+    v3 F = mk(c.fwd[0], c.fwd[1], c.fwd[2]);
+    v3 R = mk(c.right[0], c.right[1], c.right[2]);
+    v3 U = mk(c.up[0], c.up[1], c.up[2]);
+    v3 P = mk(c.pos[0], c.pos[1], c.pos[2]);
+    float a = (sx * aspect) * c.tanHalf;
+    float b = sy * c.tanHalf;
+    v3 pin = normalize((F + R * a) + U * b);
+    float lx = 0.0f, ly = 0.0f;
+    if (c.lensRadius > 0.0f) {
+        float r1 = rnd(seed);
+        float r2 = rnd(seed);
+        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+        int sector = (int)(rnd(seed) * (float)c.blades);
+        float a0 = (6.28318530718f * (float)sector) / (float)c.blades;
+        float a1 = (6.28318530718f * (float)(sector + 1)) / (float)c.blades;
+        float c0 = rc_cosf(a0), s0 = rc_sinf(a0), c1 = rc_cosf(a1), s1 = rc_sinf(a1);
+        lx = (r1 * c0 + r2 * c1) * c.lensRadius;
+        ly = (r1 * s0 + r2 * s1) * c.lensRadius;
+    }
+    float nr2 = 0.0f;
+    if (c.lensRadius > 1e-6f) nr2 = rc_clampf((lx * lx + ly * ly) / (c.lensRadius * c.lensRadius), 0.0f, 1.0f);
+    float ef = (c.focalDist + fOff) + c.saStrength * nr2;
+    ef = fmaxf(ef, 0.01f);
+    float tf = ef / dot(pin, F);
+    v3 fp = P + pin * tf;
+    v3 o = (P + R * lx) + U * ly;
+    ro = o;
+    rd = normalize(fp - o);
+}
+
+// Interpolated shading normal and material of a hit (pathtrace.comp:166-172).
+__device__ __forceinline__ v3 hit_normal(const SceneView &S, const Hit &h, int &mat) {
+    const float4 *sh = S.triShade + 3 * (size_t)h.tri;
+    float4 a = sh[0], b = sh[1], c = sh[2];
+    float w = (1.0f - h.bu) - h.bv;
+    mat = rc_f2i_bits(a.w);
+    return normalize((xyz(a) * w + xyz(b) * h.bu) + xyz(c) * h.bv);
+}
+
+__device__ __forceinline__ void wave_add_counters(unsigned long long *dst, const Counters &C) {
+    const uint32_t *c = (const uint32_t *)&C;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(Counters) / 4); ++k) {
+        uint32_t v = c[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&dst[k], (unsigned long long)v);
+    }
+}
+
+// A next-event-estimation sample whose visibility is still to be tested: if
+// the shadow ray (o, d, maxDist) is unoccluded, radiance += weight.
+struct NeeSample {
+    bool shoot;
+    v3 o, d;
+    float maxDist;
+    v3 weight;  // throughput * clamped NEE contribution (pathtrace.comp:385-394, :573)
+};
+
+// One path vertex after a valid hit (pathtrace.comp:475-581): emission, Russian
+// roulette, and the BSDF branch (glass, GGX metal, or Lambertian with NEE). On
+// return, `ro`/`rd` hold the next ray unless the path ended or this was the last
+// bounce (`last`, whose sampled direction is never traced and whose random
+// numbers are never reused, so it is not sampled). Returns true if Russian
+// roulette ended the path.
+template <bool STATS>
+__device__ __forceinline__ bool shade_vertex(const SceneView &S, int neeEnabled, float neeClamp, bool last,
+                                             int bounce, float4 m0, float4 m1, float4 m2, v3 hn, v3 hp,
+                                             v3 &ro, v3 &rd, v3 &thr, v3 &rad, uint32_t &seed,
+                                             bool &lastDiffuse, NeeSample &ns, Counters &C) {
+    ns.shoot = false;
+    v3 albedo = xyz(m0);
+    float metallic = m0.w, estr = m1.w, rough = m2.x, ior = m2.y, trans = m2.z;
+    v3 n = dot(rd, hn) < 0.0f ? hn : neg(hn);
+    if (!lastDiffuse) rad = rad + (thr * xyz(m1)) * estr;
+    if (bounce >= 2) {
+        float surv = fmaxf(albedo.x, fmaxf(albedo.y, albedo.z));
+        if (metallic > 0.5f || trans > 0.5f) surv = 0.95f;
+        surv = rc_clampf(surv, 0.01f, 0.95f);
+        if (rnd(seed) > surv) return true;
+        thr = thr / surv;
+    }
+    if (trans > 0.5f) {
+        if (!last) {
+            bool entering = dot(rd, hn) < 0.0f;
+            float eta = entering ? (1.0f / ior) : ior;
+            float cosI = fabsf(dot(rd, n));
+            float r0 = (1.0f - ior) / (1.0f + ior);
+            r0 = r0 * r0;
+            float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
+            if (rnd(seed) < F) {
+                rd = reflect(rd, n);
+            } else {
+                v3 rf = refract(rd, n, eta);
+                if (dot(rf, rf) < 0.001f) rf = reflect(rd, n);
+                rd = normalize(rf);
+            }
+            ro = hp + rd * 2e-3f;
+        }
+        thr = thr * albedo;
+        lastDiffuse = false;
+    } else if (metallic > 0.5f) {
+        if (!last) {
+            v3 wo = neg(rd);
+            float alpha = fmaxf(rough, 1e-3f);
+            v3 up = fabsf(n.y) < 0.9999f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+            v3 T = normalize(cross(up, n));
+            v3 B = cross(n, T);
+            v3 wl = mk(dot(wo, T), dot(wo, n), dot(wo, B));
+            float u1 = rnd(seed);
+            float u2 = rnd(seed);
+            v3 hl = sample_vndf(wl, alpha, u1, u2);
+            v3 hv = normalize((T * hl.x + n * hl.y) + B * hl.z);
+            rd = reflect(rd, hv);
+            if (dot(rd, n) <= 0.0f) rd = reflect(neg(wo), n);
+            float nl = fmaxf(dot(n, rd), 1e-4f);
+            float nv = fmaxf(dot(n, wo), 1e-4f);
+            float hdv = fmaxf(dot(hv, wo), 1e-4f);
+            float pw = rc_powf(1.0f - hdv, 5.0f);
+            v3 F = mk(albedo.x + (1.0f - albedo.x) * pw, albedo.y + (1.0f - albedo.y) * pw,
+                      albedo.z + (1.0f - albedo.z) * pw);
+            float g = smith_g2(nl, nv, alpha) / fmaxf(smith_g1(nv, alpha), 1e-7f);
+            thr = thr * (F * g);
+            ro = hp + n * 2e-3f;
+        }
+        lastDiffuse = false;
+    } else {
+        if (neeEnabled != 0 && S.nLights > 0 && S.totalLightPower > 0.0f) {
+            // sampleDirectLight (pathtrace.comp:341-395). An early-out or a zero
+            // pdf adds thr * 0 in the reference, which leaves radiance unchanged.
+            float us = rnd(seed);
+            int sel = S.nLights - 1;
+            for (int i = 0; i < S.nLights; ++i) {
+                if (STATS) C.lightScans++;
+                if (us < S.cdf[i]) { sel = i; break; }
+            }
+            if (STATS) C.lightReads++;
+            const float4 *L = S.lights + 5 * (size_t)sel;
+            float4 L0 = L[0], L3 = L[3];
+            float pickPdf = L3.w / S.totalLightPower;
+            if (pickPdf > 0.0f) {
+                float4 L1 = L[1], L2 = L[2], L4 = L[4];
+                float r1 = rnd(seed);
+                float r2 = rnd(seed);
+                if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+                v3 lv0 = xyz(L0);
+                v3 lp = (lv0 + (xyz(L1) - lv0) * r1) + (xyz(L2) - lv0) * r2;
+                v3 tl = lp - hp;
+                float d2 = dot(tl, tl);
+                float d = sqrtf(d2);
+                v3 sd = tl / d;
+                float cosHit = dot(n, sd);
+                float cosLight = fabsf(dot(xyz(L4), sd));
+                if (cosHit > 0.0f && cosLight > 0.0f) {
+                    float area = L0.w;
+                    float pdf = pickPdf * (fmaxf(d2, area) / (area * cosLight));
+                    if (pdf > 0.0f) {
+                        v3 em = xyz(L3);
+                        float cl = ((0.299f * em.x + 0.587f * em.y) + 0.114f * em.z) * neeClamp;
+                        v3 c = mk(fminf((((albedo.x / RC_PI_F) * em.x) * cosHit) / pdf, cl),
+                                  fminf((((albedo.y / RC_PI_F) * em.y) * cosHit) / pdf, cl),
+                                  fminf((((albedo.z / RC_PI_F) * em.z) * cosHit) / pdf, cl));
+                        ns.shoot = true;
+                        ns.o = hp + n * 2e-3f;
+                        ns.d = sd;
+                        ns.maxDist = d - 4e-3f;
+                        ns.weight = thr * c;
+                    }
+                }
+            }
+        }
+        if (!last) {
+            float u1 = rnd(seed);
+            float u2 = rnd(seed);
+            rd = sample_cosine(u1, u2, n);
+            ro = hp + n * 1e-3f;
+        }
+        thr = thr * albedo;
+        lastDiffuse = true;
+    }
+    return false;
+}
+
+}  // namespace rc

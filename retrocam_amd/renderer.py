@@ -110,6 +110,9 @@ class Renderer:
     def set_accum_mode(self, mode: int):
         self.ctx.check(self._lib.rc_set_accum_mode(self.ctx.h, int(mode)))
 
+    def set_pt_mode(self, mode: int):
+        self.ctx.check(self._lib.rc_set_pt_mode(self.ctx.h, int(mode)))
+
     _DT = {N.RC_BUF_ACCUM: (np.float32, 4), N.RC_BUF_GBUFFER: (np.uint16, 4), N.RC_BUF_GALBEDO: (np.uint16, 4),
            N.RC_BUF_VARIANCE: (np.float32, 1), N.RC_BUF_PRIMARY_ID: (np.int32, 1),
            N.RC_BUF_GBUFFER_ID: (np.int32, 1), N.RC_BUF_ACCUM64: (np.float64, 4),

@@ -26,9 +26,13 @@ def rel_mse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
 
 
-def gpu_render(ctx, scene, cam, params, spp, batch=1, fp64=False):
+MODES = [pytest.param(N.RC_PT_MEGAKERNEL, id="megakernel"), pytest.param(N.RC_PT_WAVEFRONT, id="wavefront")]
+
+
+def gpu_render(ctx, scene, cam, params, spp, batch=1, fp64=False, mode=N.RC_PT_MEGAKERNEL):
     SceneUploader(ctx).upload(scene)
     r = Renderer(ctx)
+    r.set_pt_mode(mode)
     r.set_accum_mode(N.RC_ACCUM_FP64 if fp64 else N.RC_ACCUM_FP32)
     r.reset()
     done = 0
@@ -122,18 +126,20 @@ def test_stack_caps_match_oracle(gpu_ctx):
     assert un[0] == 1 and rco.shadow_ray(rco.OracleScene(sc), ro[0], rd[0], 1000.0)
 
 
-def test_cornell_small_parity(gpu_ctx, cornell):
+@pytest.mark.parametrize("mode", MODES)
+def test_cornell_small_parity(mode, gpu_ctx, cornell):
     w = cornell
-    g = gpu_render(gpu_ctx, w.scene, w.camera, w.params, w.spp)
+    g = gpu_render(gpu_ctx, w.scene, w.camera, w.params, w.spp, batch=w.spp, mode=mode)
     o = oracle_render(w.scene, w.camera, w.params, w.spp)
     assert_parity(g, o)
 
 
-def test_thin_lens_sa_lca_parity(gpu_ctx, cornell):
+@pytest.mark.parametrize("mode", MODES)
+def test_thin_lens_sa_lca_parity(mode, gpu_ctx, cornell):
     w = cornell
     cam = CameraSpec(**{**w.camera.__dict__, "pinhole": False, "fStop": 1.4, "blades": 6, "saStrength": 0.04,
                         "focalDist": 9.0, "lcaDelta": (-0.2, 0.0, 0.3), "width": 96, "height": 54})
-    g = gpu_render(gpu_ctx, w.scene, cam, w.params, 3)
+    g = gpu_render(gpu_ctx, w.scene, cam, w.params, 3, batch=2, mode=mode)
     o = oracle_render(w.scene, cam, w.params, 3)
     assert_parity(g, o)
 
@@ -141,30 +147,35 @@ def test_thin_lens_sa_lca_parity(gpu_ctx, cornell):
 def test_batched_passes_equal_single_passes(gpu_ctx, cornell):
     w = cornell
     cam = CameraSpec(**{**w.camera.__dict__, "width": 128, "height": 72})
-    a = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=1)
-    b = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=4)
+    a = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=1, mode=N.RC_PT_MEGAKERNEL)
+    b = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=4, mode=N.RC_PT_WAVEFRONT)
+    c = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=3, mode=N.RC_PT_MEGAKERNEL)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
 
 
-def test_fp64_accumulation_parity(gpu_ctx, cornell):
+@pytest.mark.parametrize("mode", MODES)
+def test_fp64_accumulation_parity(mode, gpu_ctx, cornell):
     w = cornell
     cam = CameraSpec(**{**w.camera.__dict__, "width": 64, "height": 36})
-    g = gpu_render(gpu_ctx, w.scene, cam, w.params, 5, batch=2, fp64=True)
+    g = gpu_render(gpu_ctx, w.scene, cam, w.params, 5, batch=2, fp64=True, mode=mode)
     o = oracle_render(w.scene, cam, w.params, 5, fp64=True)
     assert np.array_equal(g["accum64"], o.accum64)
     assert np.array_equal(g["accum"], o.accum64.astype(np.float32))
 
 
-def test_full_frame_parity_subset(gpu_ctx, full_frame_scene):
+@pytest.mark.parametrize("mode", MODES)
+def test_full_frame_parity_subset(mode, gpu_ctx, full_frame_scene):
     # 854x480 thin-lens frame over 250k triangles; oracle on every 4th pixel.
     w = full_frame_scene
-    g = gpu_render(gpu_ctx, w.scene, w.camera, w.params, 2, batch=2)
+    g = gpu_render(gpu_ctx, w.scene, w.camera, w.params, 2, batch=2, mode=mode)
     o = oracle_render(w.scene, w.camera, w.params, 2, stride=4)
     assert_parity(g, o, mask=o.mask())
 
 
-def test_zero_lights_and_small_leaves(gpu_ctx):
+@pytest.mark.parametrize("mode", MODES)
+def test_zero_lights_and_small_leaves(mode, gpu_ctx):
     b = SceneBuilder()
     m = b.add_material(Material.diffuse(0.6, 0.5, 0.4))
     g = b.add_material(Material.glass(1.5))
@@ -175,7 +186,7 @@ def test_zero_lights_and_small_leaves(gpu_ctx):
     cam = CameraSpec(eye=(0.1, 2.0, 0.2), forward=(0.0, -1.0, 0.0), right=(1.0, 0.0, 0.0), up=(0.0, 0.0, -1.0),
                      width=32, height=32, pinhole=True)
     p = PtParams(1, 1.0, 6)
-    gg = gpu_render(gpu_ctx, sc, cam, p, 3)
+    gg = gpu_render(gpu_ctx, sc, cam, p, 3, batch=3, mode=mode)
     o = oracle_render(sc, cam, p, 3)
     assert_parity(gg, o)
 
