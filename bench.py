@@ -2,12 +2,18 @@
 
 Workload (default): the full plain frame — 854x480, 128 spp per frame, 6
 bounces, NEE, thin lens 35 mm f/2.8 with hexagon bokeh, over a seeded
-250,000-triangle scene (retrocam_amd.workloads.full_frame). One step = one
-frame: reset, then 128 progressive samples in one launch of the path-trace
-kernel (the same arithmetic as 128 Renderer.render passes).
+250,000-triangle scene (retrocam_amd.workloads.full_frame), then the post
+chain: p00 normalize, 5 a-trous passes, all 14 VHS passes and the display
+encode. One step = one frame: reset, 128 progressive samples in one launch of
+the path-trace kernel (the same arithmetic as 128 Renderer.render passes),
+then the post chain (RenderPipeline.accumulateFrame without SPPM,
+src/main/java/com/retrocam/export/RenderPipeline.java:209-239).
 
-Prints ONE JSON line (rank 0). See DESIGN.md for the byte model behind
-"roofline" and for the CPU baseline sample.
+Quality: relMSE = mean over pixels and RGB of (x - r)^2 / (r^2 + 0.01) on linear
+HDR after p00 (+ a-trous), against r = a 4096-spp FP64-accumulated render of
+the same frame by the same path tracer.
+
+Prints ONE JSON line.
 """
 import argparse
 import ctypes as C
@@ -36,6 +42,8 @@ BYTE_MODEL = {
     "lightReads": 80,      # selected Light record (:350)
 }
 PIXEL_BYTES_PER_SPP = 40   # accum RGBA32F read+write (32) + variance R32F read+write (8) (:649-653)
+# Post chain, compulsory bytes per pixel: accum 16 + gBuffer 8 + gAlbedo 8 + variance 4 + RGBA8 out 4.
+POST_COMPULSORY_BYTES = 40
 
 
 class HipTimer:
@@ -97,24 +105,48 @@ def cpu_baseline(w, spp_sample: int):
             "mrays_per_s": rays / dt / 1e6}
 
 
+def read_traffic():
+    """HBM bytes per path-trace launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "pathtrace_kernel__full_frame__pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def rel_mse(x, r):
+    return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=128)
-    ap.add_argument("--cpu-spp", type=int, default=4, help="samples timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
+    ap.add_argument("--sweep", default="16,32,64,128,256,512",
+                    help="spp values searched for the relMSE <= 0.01 operating point ('' = skip)")
+    ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["wavefront", "megakernel"], default="megakernel")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run 1 GiB device copies (calibrates FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc)")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the render path runs on a single MI355X "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the render path runs on a single MI355X; "
+              "splitting a frame over GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
+    from retrocam_amd import _native as N
     from retrocam_amd import workloads
+    from retrocam_amd.post import VHS_PASSES, PostProcessStack, TemporalInputs, make_params
     from retrocam_amd.renderer import COUNTER_FIELDS, RenderContext, Renderer, SceneUploader
 
     def log(*a):
@@ -127,42 +159,105 @@ def main():
     ctx = RenderContext(0)
     SceneUploader(ctx).upload(w.scene)
     r = Renderer(ctx)
-    from retrocam_amd import _native as N
     r.set_pt_mode(N.RC_PT_WAVEFRONT if args.pt_mode == "wavefront" else N.RC_PT_MEGAKERNEL)
+    post = PostProcessStack(ctx)
+    pp_full = make_params(enable=VHS_PASSES, atrous=True, iterations=5)
+    pp_denoise = make_params(atrous=True, iterations=5)
+    pp_raw = make_params()
+    temporal = TemporalInputs(time=0.0, agcGain=1.0, frameIndex=0)
+    npx = w.camera.width * w.camera.height
 
-    def frame():
+    def trace(spp):
         r.reset()
-        r.render(w.camera, w.params, n_samples=w.spp)
+        r.render(w.camera, w.params, n_samples=spp)
 
     # Counting pass (separately compiled instrumented kernel, identical results):
     r.debug_counters(True)
-    frame()
+    trace(w.spp)
     ctx.sync()
     cnt = dict(zip(COUNTER_FIELDS, (int(v) for v in r.debug_counters(False))))
-    npx = w.camera.width * w.camera.height
     alg_bytes = sum(BYTE_MODEL[k] * cnt[k] for k in BYTE_MODEL) + PIXEL_BYTES_PER_SPP * npx * w.spp
     rays = cnt["rays"] + cnt["shadowRays"]
     log(f"[bench] counters per frame: {cnt}")
 
     for _ in range(args.warmup):
-        frame()
+        trace(w.spp)
+        post.run_on_accum(pp_full, temporal, read=False)
     ctx.sync()
     torch.cuda.synchronize()
-    timer = HipTimer(ctx.stream)
+    t_trace = HipTimer(ctx.stream)
+    t_post = HipTimer(ctx.stream)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        timer.start()
-        frame()
-        timer.stop()
+        t_trace.start()
+        trace(w.spp)
+        t_trace.stop()
+        t_post.start()
+        post.run_on_accum(pp_full, temporal, read=False)
+        t_post.stop()
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kms = timer.times_ms()
+    kms = t_trace.times_ms()
+    pms = t_post.times_ms()
     ms_per_step = wall / args.steps * 1e3
     fps = args.steps / wall
     avg_kernel_s = float(np.mean(kms)) / 1e3
     achieved = alg_bytes / avg_kernel_s / 1e9
-    log(f"[bench] per-frame event times (ms): {[round(x, 2) for x in kms]}")
+    post_ms = float(np.mean(pms))
+    log(f"[bench] per-frame trace ms: {[round(x, 2) for x in kms]}; post chain ms: {[round(x, 3) for x in pms]}")
+
+    if args.copy_calib:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 3 x 1 GiB device-to-device copies done")
+
+    quality = None
+    if args.ref_spp > 0:
+        # the 128-spp frame of the timed loop is still in the accumulation buffer
+        post.run_on_accum(pp_raw, temporal, read=False)
+        raw = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
+        post.run_on_accum(pp_denoise, temporal, read=False)
+        den = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
+        r.set_accum_mode(N.RC_ACCUM_FP64)
+        r.reset()
+        done = 0
+        t1 = time.perf_counter()
+        while done < args.ref_spp:
+            k = min(256, args.ref_spp - done)
+            r.render(w.camera, w.params, n_samples=k)
+            done += k
+            ctx.sync()
+            log(f"[bench] reference: {done}/{args.ref_spp} spp ({time.perf_counter() - t1:.1f} s)")
+        acc64 = r.read(N.RC_BUF_ACCUM64)
+        ref = acc64[:, :3] / acc64[:, 3:4] * float(pp_full.exposure)
+        r.set_accum_mode(N.RC_ACCUM_FP32)
+        quality = {"reference_spp": args.ref_spp, "relmse_raw": rel_mse(raw, ref), "relmse_atrous": rel_mse(den, ref),
+                   "target": 0.01}
+        sweep = [int(v) for v in args.sweep.split(",") if v.strip()]
+        curve = []
+        for spp in sweep:
+            trace(spp)
+            post.run_on_accum(pp_denoise, temporal, read=False)
+            d = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
+            curve.append([spp, round(rel_mse(d, ref), 6)])
+        quality["atrous_curve"] = curve
+        ok = [s for s, e in curve if e <= 0.01]
+        if ok:
+            s_star = min(ok)
+            tq = HipTimer(ctx.stream)
+            for _ in range(3):
+                tq.start()
+                trace(s_star)
+                post.run_on_accum(pp_full, temporal, read=False)
+                tq.stop()
+            quality["operating_point"] = {"spp": s_star, "frames_per_s": round(1e3 / float(np.mean(tq.times_ms())), 4),
+                                          "method": "path tracer + a-trous x5 + VHS"}
+        log(f"[bench] quality: {quality}")
 
     cpu = None
     if args.cpu_spp > 0:
@@ -170,7 +265,7 @@ def main():
         cpu = cpu_baseline(w, args.cpu_spp)
 
     out = {
-        "metric": "854x480 frames/s (plain path tracer, 128 spp/frame); Mrays/s",
+        "metric": "854x480 frames/s at relMSE<=0.01 vs 4096spp (net vs a-trous); Mrays/s",
         "value": round(fps, 4),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -182,15 +277,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded 250k-triangle scene, generated in-process)",
-        "config": {"workload": "full_frame", "width": w.camera.width, "height": w.camera.height,
+        "config": {"workload": "full plain frame: 250k tris, 854x480, 128 spp, 6 bounces, a-trous x5, 14 VHS passes",
+                   "width": w.camera.width, "height": w.camera.height,
                    "spp": w.spp, "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
                    "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU",
                    "pt_mode": args.pt_mode},
-        "mrays_per_s": round(rays * fps / 1e6, 2),
+        "mrays_per_s": round(rays * 1e3 / float(np.mean(kms)) / 1e6, 2),
         "rays_per_frame": rays,
+        "trace_ms": round(float(np.mean(kms)), 3),
+        "post_ms": round(post_ms, 4),
+        "post_gbs_compulsory": round(POST_COMPULSORY_BYTES * npx / (post_ms / 1e3) / 1e9, 1),
+        "quality": quality,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "pathtrace_kernel", "alg_bytes_per_launch": int(alg_bytes),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                     "kernel": "pathtrace_kernel" if args.pt_mode == "megakernel" else "wavefront frame",
+                     "alg_bytes_per_launch": int(alg_bytes),
                      "avg_launch_ms": round(avg_kernel_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
