@@ -170,6 +170,13 @@ enum {
     RC_PT_WAVEFRONT = 1     /* extend / shadow / shade kernels over ray queues    */
 };
 
+/* Denoiser of the post chain (rc_set_denoiser). */
+enum {
+    RC_DENOISE_ATROUS = 0,  /* the reference's a-trous filter (default)          */
+    RC_DENOISE_NET = 1      /* the learned U-Net (rc_net_*), a-trous kept as the
+                               fallback when no network is loaded                */
+};
+
 typedef struct rc_ctx rc_ctx;
 
 /* ---- context ------------------------------------------------------------ */
@@ -263,6 +270,33 @@ int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t);
  * host RGBA32F image (rows bottom-up) with N = 1, exposure 1 and no a-trous. */
 int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
                   const rc_temporal *t);
+
+/* ---- learned denoiser (no counterpart in the reference) ------------------ */
+/* A 5-level U-Net (3x3 convolutions, 32..128 channels) on bfloat16 MFMA with
+ * FP32 accumulation, trained on the GPU by its own forward/backward kernels.
+ * Inputs per pixel ("raw", 10 floats): radiance rgb, albedo rgb, normal xyz,
+ * depth — the render outputs of pathtrace.comp:6-7, 644-647. Targets: 3 floats
+ * (high-spp radiance). Images are n x h x w, h and w multiples of 16. The flat
+ * parameter vector holds, per layer, W[cout][ky][kx][cin] then b[cout]. */
+int32_t rc_net_param_count(void);
+/* He-normal initialisation from a seed; resets the optimizer state. */
+int rc_net_init(rc_ctx *ctx, uint32_t seed);
+int rc_net_set_params(rc_ctx *ctx, const float *params, int32_t count);
+int rc_net_get_params(rc_ctx *ctx, float *params, int32_t count);
+/* Copies a training set to the device: raw n x h x w x 10, target n x h x w x 3. */
+int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32_t n, int32_t h, int32_t w);
+/* `steps` Adam steps (b1 0.9, b2 0.999) on batches of `batch` crops drawn
+ * from the dataset with random flips; loss = mean (x-r)^2/(r^2+0.01). Writes
+ * the loss of every step to losses[steps] (may be NULL). */
+int rc_net_train(rc_ctx *ctx, int32_t steps, int32_t batch, float lr, uint32_t seed, float *losses);
+/* One forward + backward pass without an update: loss and the gradient
+ * (rc_net_param_count floats). */
+int rc_net_grad(rc_ctx *ctx, const float *raw, const float *target, int32_t n, int32_t h, int32_t w, float *loss,
+                float *grad);
+/* Network output (linear radiance) for raw inputs: out n x h x w x 3. */
+int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t w, float *out);
+/* Which denoiser rc_post runs when rc_post_params.atrousEnabled is set. */
+int rc_set_denoiser(rc_ctx *ctx, int32_t mode);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Work counters of the path-trace kernel, for the roofline byte model. Copies

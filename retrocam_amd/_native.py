@@ -39,6 +39,9 @@ RC_ACCUM_FP64 = 1
 RC_PT_MEGAKERNEL = 0
 RC_PT_WAVEFRONT = 1
 
+RC_DENOISE_ATROUS = 0
+RC_DENOISE_NET = 1
+
 F3 = C.c_float * 3
 
 
@@ -123,6 +126,15 @@ HIP_SYMBOLS = {
     "rc_post_defaults": (None, [C.POINTER(RcPostParams)]),
     "rc_post": (C.c_int, [_VP, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
     "rc_post_image": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
+    "rc_net_param_count": (C.c_int32, []),
+    "rc_net_init": (C.c_int, [_VP, C.c_uint32]),
+    "rc_net_set_params": (C.c_int, [_VP, _FP, C.c_int32]),
+    "rc_net_get_params": (C.c_int, [_VP, _FP, C.c_int32]),
+    "rc_net_set_dataset": (C.c_int, [_VP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32]),
+    "rc_net_train": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _FP]),
+    "rc_net_grad": (C.c_int, [_VP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP, _FP]),
+    "rc_net_forward": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP]),
+    "rc_set_denoiser": (C.c_int, [_VP, C.c_int32]),
     "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),
     "rc_debug_pcg": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
     "rc_debug_rays": (C.c_int, [_VP, _FP, C.c_int32, C.POINTER(C.c_int32), _FP, C.POINTER(C.c_int32)]),

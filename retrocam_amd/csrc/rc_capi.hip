@@ -78,7 +78,11 @@ static int run_post_chain(rc_ctx *ctx, const float4 *src, int w, int h, const rc
                                     nullptr);
     if (e != hipSuccess) return hip_fail(ctx, e, "post p00 launch");
     bool exceeded = p.atrousMaxSpp > 0 && total > p.atrousMaxSpp;
-    if (p.atrousEnabled && withGbuf && !exceeded) {
+    if (p.atrousEnabled && withGbuf && !exceeded && ctx->denoiser == RC_DENOISE_NET && net_ready(ctx)) {
+        int rc = run_net_denoise(ctx, a, b, w, h);
+        if (rc) return rc;
+        float4 *tmp = a; a = b; b = tmp;
+    } else if (p.atrousEnabled && withGbuf && !exceeded) {
         int np = p.atrousIterations < 1 ? 1 : (p.atrousIterations > 5 ? 5 : p.atrousIterations);
         for (int i = 0; i < np; ++i) {  // AtrousDenoiser.denoise (AtrousDenoiser.java:60-125)
             e = launch_post_pass(ctx->stream, 100, a, b, w, h, p, t, im.gbuf, im.galbedo, im.variance, total, 1 << i,
@@ -220,6 +224,7 @@ int rc_destroy(rc_ctx *ctx) {
     free_images(ctx->img);
     free_sppm(ctx->sppm);
     free_wavefront(ctx->wf);
+    free_net(ctx->net);
     dfree(ctx->counters);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -91,6 +91,8 @@ struct Wavefront {
     int *hostCnt = nullptr;  // pinned
 };
 
+struct Net;  // learned denoiser (rc_net.hip)
+
 }  // namespace rc
 
 struct rc_ctx {
@@ -104,6 +106,8 @@ struct rc_ctx {
     rc::Sppm sppm;
     rc::Wavefront wf;
     int32_t ptMode = RC_PT_MEGAKERNEL;
+    rc::Net *net = nullptr;
+    int32_t denoiser = RC_DENOISE_ATROUS;
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
     int countersEnabled = 0;
 };
@@ -122,6 +126,12 @@ void free_sppm(Sppm &s);
 // Wavefront form of launch_pathtrace (rc_wavefront.hip); same results.
 hipError_t launch_wavefront(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
 void free_wavefront(Wavefront &w);
+
+// Learned denoiser (rc_net.hip).
+void free_net(Net *n);
+bool net_ready(const rc_ctx *ctx);
+// Denoises the w x h RGBA32F post image src into dst with the frame's G-buffers.
+int run_net_denoise(rc_ctx *ctx, const float4 *src, float4 *dst, int w, int h);
 
 // One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
 // with the given step, 200 = display encode into rgba8.

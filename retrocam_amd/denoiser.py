@@ -1,0 +1,154 @@
+"""Learned denoiser over the C ABI (rc_net_* in include/retrocam_hip.h).
+
+Project-new: the reference denoises only with the a-trous filter
+(src/main/java/com/retrocam/post/AtrousDenoiser.java:60-125), which stays the
+fallback. The network's per-pixel inputs are the reference's render outputs
+(pathtrace.comp:6-7, 644-647): the accumulated radiance mean, first-hit albedo,
+shading normal and hit distance ("raw", 10 floats per pixel).
+
+  NetDenoiser     parameters, training, forward, and the switch that makes
+                  rc_post run the network instead of a-trous
+  frame_raw       raw inputs of the frame currently in a Renderer
+  render_pair     (raw at low spp, radiance target at high spp) of one view
+  make_dataset    pairs over seeded training scenes (workloads.training_scene)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native as N
+from .renderer import Renderer, SceneUploader
+
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_denoiser.npy")
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def param_count() -> int:
+    return int(N.hip_lib().rc_net_param_count())
+
+
+class NetDenoiser:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.n = int(self._lib.rc_net_param_count())
+
+    def init(self, seed: int = 1) -> "NetDenoiser":
+        self.ctx.check(self._lib.rc_net_init(self.ctx.h, int(seed) & 0xFFFFFFFF))
+        return self
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.n, np.float32)
+        self.ctx.check(self._lib.rc_net_get_params(self.ctx.h, _fp(out), self.n))
+        return out
+
+    def set_params(self, theta: np.ndarray) -> "NetDenoiser":
+        t = np.ascontiguousarray(theta, np.float32)
+        self.ctx.check(self._lib.rc_net_set_params(self.ctx.h, _fp(t), int(t.size)))
+        return self
+
+    def save(self, path: str = WEIGHTS_PATH):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.save(path, self.get_params())
+
+    def load(self, path: str = WEIGHTS_PATH) -> "NetDenoiser":
+        return self.set_params(np.load(path, allow_pickle=False))
+
+    def set_dataset(self, raw: np.ndarray, target: np.ndarray):
+        raw = np.ascontiguousarray(raw, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n, h, w, c = raw.shape
+        assert c == 10 and target.shape == (n, h, w, 3)
+        self.ctx.check(self._lib.rc_net_set_dataset(self.ctx.h, _fp(raw), _fp(target), n, h, w))
+
+    def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+        losses = np.zeros(steps, np.float32)
+        self.ctx.check(self._lib.rc_net_train(self.ctx.h, int(steps), int(batch), float(lr), int(seed) & 0xFFFFFFFF,
+                                              _fp(losses)))
+        return losses
+
+    def grad(self, raw: np.ndarray, target: np.ndarray):
+        raw = np.ascontiguousarray(raw, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n, h, w, _ = raw.shape
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n, np.float32)
+        self.ctx.check(self._lib.rc_net_grad(self.ctx.h, _fp(raw), _fp(target), n, h, w, _fp(loss), _fp(g)))
+        return float(loss[0]), g
+
+    def forward(self, raw: np.ndarray) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, np.float32)
+        n, h, w, _ = raw.shape
+        out = np.zeros((n, h, w, 3), np.float32)
+        self.ctx.check(self._lib.rc_net_forward(self.ctx.h, _fp(raw), n, h, w, _fp(out)))
+        return out
+
+    def enable(self, on: bool = True):
+        """rc_post's denoise step runs the network (on) or a-trous (off)."""
+        self.ctx.check(self._lib.rc_set_denoiser(self.ctx.h, N.RC_DENOISE_NET if on else N.RC_DENOISE_ATROUS))
+
+
+def _half(a: np.ndarray) -> np.ndarray:
+    return a.view(np.float16).astype(np.float32)
+
+
+def frame_raw(r: Renderer, width: int, height: int) -> np.ndarray:
+    """(height, width, 10) raw inputs of the frame in r: radiance mean, albedo,
+    normal, depth (bottom-up rows, as the reference stores images)."""
+    acc = r.read(N.RC_BUF_ACCUM)
+    gb = _half(r.read(N.RC_BUF_GBUFFER))
+    ga = _half(r.read(N.RC_BUF_GALBEDO))
+    raw = np.zeros((height * width, 10), np.float32)
+    raw[:, 0:3] = acc[:, 0:3] / acc[:, 3:4]
+    raw[:, 3:6] = ga[:, 0:3]
+    raw[:, 6:9] = gb[:, 0:3]
+    raw[:, 9] = gb[:, 3]
+    return raw.reshape(height, width, 10)
+
+
+def render_pair(ctx, w, spp_in: int, spp_ref: int):
+    """Raw inputs at spp_in and the FP64-accumulated spp_ref radiance target of workload w."""
+    SceneUploader(ctx).upload(w.scene)
+    r = Renderer(ctx)
+    r.set_accum_mode(N.RC_ACCUM_FP32)
+    r.reset()
+    r.render(w.camera, w.params, n_samples=spp_in)
+    raw = frame_raw(r, w.camera.width, w.camera.height)
+    r.set_accum_mode(N.RC_ACCUM_FP64)
+    r.reset()
+    done = 0
+    while done < spp_ref:
+        k = min(512, spp_ref - done)
+        r.render(w.camera, w.params, n_samples=k)
+        done += k
+    acc = r.read(N.RC_BUF_ACCUM64)
+    tgt = (acc[:, 0:3] / acc[:, 3:4]).astype(np.float32).reshape(w.camera.height, w.camera.width, 3)
+    r.set_accum_mode(N.RC_ACCUM_FP32)
+    return raw, tgt
+
+
+def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32), spp_ref: int = 4096, log=None):
+    """Pairs over workloads.training_scene(seed); spp_in cycles through the given counts."""
+    from . import workloads
+    raws, tgts = [], []
+    for i, s in enumerate(seeds):
+        w = workloads.training_scene(int(s), size=size)
+        raw, tgt = render_pair(ctx, w, int(spp_in[i % len(spp_in)]), spp_ref)
+        raws.append(raw)
+        tgts.append(tgt)
+        if log and (i + 1) % 10 == 0:
+            log(f"[dataset] {i + 1}/{len(seeds)} scenes")
+    return np.stack(raws), np.stack(tgts)
+
+
+def rel_mse(x: np.ndarray, r: np.ndarray) -> float:
+    """mean over pixels and RGB of (x - r)^2 / (r^2 + 0.01)."""
+    x = np.asarray(x, np.float64)
+    r = np.asarray(r, np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 0.01)))

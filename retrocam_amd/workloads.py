@@ -7,6 +7,8 @@
                   (50% diffuse, 30% GGX metal alpha U[0.05, 0.6], 20% glass
                   ior 1.5) in a closed room with 8 quad area lights; thin lens
                   35 mm f/2.8, hexagon bokeh, focus 2.0 m, SA 0.04.
+  training_scene  seeded size x size denoiser-training views (rooms with random
+                  objects, lights and lenses).
   reference_default  the reference's default Cornell scene
                   (src/main/java/com/retrocam/scene/SceneEditor.java:111-134) at 854x480.
 """
@@ -138,6 +140,58 @@ def sppm_caustics(seed: int = 1, width: int = 854, height: int = 480, spp: int =
     return Workload("sppm_caustics", scene, w.camera, w.params, spp,
                     {"seed": seed, "photons": photons, "hash_size": hash_size, "init_radius": init_radius,
                      "alpha": alpha})
+
+
+def training_scene(seed: int, size: int = 128, spp: int = 8) -> Workload:
+    """A seeded denoiser-training view (size x size): a closed room with random
+    wall albedos, 3-10 jittered spheres/boxes with the full-frame material mix
+    (50% diffuse, 30% GGX metal, 20% glass), 1-3 quad lights of random strength
+    and a random thin-lens or pinhole camera looking into the room."""
+    rng = np.random.default_rng(100_000 + seed)
+    b = SceneBuilder()
+    walls = [b.add_material(Material.diffuse(*rng.uniform(0.1, 0.8, 3))) for _ in range(6)]
+    X, Y, Z = float(rng.uniform(4, 8)), float(rng.uniform(4, 7)), float(rng.uniform(4, 8))
+    b.add_quad((-X, 0, -Z), (-X, 0, Z), (X, 0, Z), (X, 0, -Z), walls[0])
+    b.add_quad((-X, Y, -Z), (X, Y, -Z), (X, Y, Z), (-X, Y, Z), walls[1])
+    b.add_quad((-X, 0, -Z), (X, 0, -Z), (X, Y, -Z), (-X, Y, -Z), walls[2])
+    b.add_quad((-X, 0, Z), (-X, Y, Z), (X, Y, Z), (X, 0, Z), walls[3])
+    b.add_quad((-X, 0, -Z), (-X, Y, -Z), (-X, Y, Z), (-X, 0, Z), walls[4])
+    b.add_quad((X, 0, -Z), (X, 0, Z), (X, Y, Z), (X, Y, -Z), walls[5])
+    for _ in range(int(rng.integers(1, 4))):
+        lm = b.add_material(Material.emissive(*rng.uniform(0.7, 1.0, 3), float(rng.uniform(5.0, 30.0))))
+        cx, cz = rng.uniform(-0.7 * X, 0.7 * X), rng.uniform(-0.7 * Z, 0.7 * Z)
+        hx, hz = rng.uniform(0.3, 1.2, 2)
+        ly = Y - 0.01
+        b.add_quad((cx - hx, ly, cz - hz), (cx + hx, ly, cz - hz), (cx + hx, ly, cz + hz), (cx - hx, ly, cz + hz), lm)
+
+    def material():
+        u = rng.random()
+        if u < 0.5:
+            return b.add_material(Material.diffuse(*rng.uniform(0.1, 0.8, 3)))
+        if u < 0.8:
+            return b.add_material(Material.metal(*rng.uniform(0.5, 0.95, 3), float(rng.uniform(0.05, 0.6))))
+        return b.add_material(Material.glass(1.5))
+
+    for _ in range(int(rng.integers(3, 11))):
+        x, z = rng.uniform(-0.8 * X, 0.8 * X), rng.uniform(-0.8 * Z, 0.8 * Z)
+        r = float(rng.uniform(0.3, 1.2))
+        y = r + float(rng.uniform(0.0, 1.5)) * (rng.random() < 0.3)
+        if rng.random() < 0.5:
+            b.add_sphere((x, y, z), r, material(), stacks=16, slices=32)
+        else:
+            rot = tuple(float(a) for a in rng.uniform(0.0, 90.0, 3))
+            b.add_box((x, y, z), (r, r * float(rng.uniform(0.5, 1.5)), r), material(), rot_deg=rot)
+    scene = b.build()
+    eye = (float(rng.uniform(-0.6 * X, 0.6 * X)), float(rng.uniform(0.8, 0.8 * Y)), float(0.9 * Z))
+    tgt = (float(rng.uniform(-0.3 * X, 0.3 * X)), float(rng.uniform(0.3, 0.5 * Y)), float(rng.uniform(-0.5 * Z, 0.2 * Z)))
+    if rng.random() < 0.6:
+        dist = float(np.linalg.norm(np.subtract(eye, tgt)))
+        cam = lookat_camera(size, size, eye=eye, target=tgt, focalDist=float(dist * rng.uniform(0.5, 1.2)),
+                            focalLengthMm=35.0, fStop=float(rng.choice([1.4, 2.0, 2.8, 5.6])),
+                            blades=int(rng.integers(5, 8)), saStrength=0.04)
+    else:
+        cam = lookat_camera(size, size, eye=eye, target=tgt, pinhole=True)
+    return Workload("training_scene", scene, cam, PtParams(1, 1.0, 6), spp, {"seed": seed})
 
 
 def reference_default(width: int = 854, height: int = 480, spp: int = 16) -> Workload:

@@ -1,0 +1,79 @@
+"""CPU checks of the FP64 U-Net oracle (oracle/rc_oracle_unet.py): its analytic
+gradient against central finite differences, the adjoint pairs it uses, the
+bfloat16 rounding, and that its parameter layout matches the library's."""
+import numpy as np
+
+from oracle import rc_oracle_unet as U
+from retrocam_amd import _native as N
+
+
+def small_problem(seed=0, n=1, h=16, w=16):
+    rng = np.random.default_rng(seed)
+    f = U.features(rng.uniform(0, 3, (n, h, w, 3)), rng.uniform(0, 1, (n, h, w, 3)),
+                   rng.standard_normal((n, h, w, 3)), rng.uniform(0, 10, (n, h, w)))
+    tgt = rng.uniform(0, 3, (n, h, w, 3))
+    return f, tgt
+
+
+def test_param_count_matches_library():
+    assert U.param_count() == 482787
+    assert N.hip_lib().rc_net_param_count() == U.param_count()
+
+
+def test_gradient_matches_finite_differences():
+    f, tgt = small_problem()
+    th = U.he_init(1) * 0.5
+    _, g = U.loss_and_grad(th, f, tgt)
+    rng = np.random.default_rng(3)
+    errs, off = [], 0
+    for _, ws, bs in U.layer_shapes():
+        nw = int(np.prod(ws))
+        for i in list(off + rng.integers(0, nw, 4)) + [off + nw + int(rng.integers(0, bs[0]))]:
+            e = 1e-6
+            tp, tm = th.copy(), th.copy()
+            tp[i] += e
+            tm[i] -= e
+            fd = (U.loss_and_grad(tp, f, tgt)[0] - U.loss_and_grad(tm, f, tgt)[0]) / (2 * e)
+            errs.append(abs(fd - g[i]) / max(abs(fd), abs(g[i]), 1e-6))
+        off += nw + bs[0]
+    errs = np.array(errs)
+    # a ReLU kink inside +-1e-6 may spoil a sample; the bulk must agree to 1e-5
+    assert np.median(errs) < 1e-6
+    assert np.mean(errs < 1e-5) >= 0.9
+
+
+def test_head_bias_gradient_is_exact():
+    # the head bias gradient is the sum of dL/dy; a halved value would be caught
+    f, tgt = small_problem(1)
+    th = U.he_init(2) * 0.5
+    _, g = U.loss_and_grad(th, f, tgt)
+    p = U.unpack(th)
+    # the last-layer bias gradient equals the sum of dL/dy: check against FD
+    off = sum(int(np.prod(ws)) + bs[0] for _, ws, bs in U.layer_shapes()[:-1]) + 96
+    e = 1e-6
+    tp, tm = th.copy(), th.copy()
+    tp[off] += e
+    tm[off] -= e
+    fd = (U.loss_and_grad(tp, f, tgt)[0] - U.loss_and_grad(tm, f, tgt)[0]) / (2 * e)
+    assert abs(fd - g[off]) < 1e-6 * max(1.0, abs(fd))
+    assert abs(fd - 0.5 * g[off]) > 1e-3 * abs(fd)  # a halved gradient would be caught
+    assert p["out"][0].shape == (3, 1, 1, 32)
+
+
+def test_pool_up_adjoints():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 8, 6, 4))
+    y = rng.standard_normal((2, 4, 3, 4))
+    assert np.isclose(np.sum(U.pool(x) * y), np.sum(x * U.pool_backward(y)))
+    assert np.isclose(np.sum(U.up(y) * x), np.sum(y * U.up_backward(x)))
+
+
+def test_bf16_rounding():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 3.0e38, 1e-40])
+    r = U.bf16(v)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                      # tie -> even
+    assert r[2] == 1.0 + 2 ** -6            # tie -> even (up)
+    assert r[3] == 1.0 + 2 ** -7            # exact
+    assert r[4] == -2.5
+    assert np.all(np.abs(r[:5] - v[:5]) <= np.abs(v[:5]) * 2 ** -8)
