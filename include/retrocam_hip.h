@@ -179,6 +179,9 @@ enum {
 
 typedef struct rc_ctx rc_ctx;
 
+/* Scene slots for batched views (slot 0 is the scene of rc_upload_scene). */
+#define RC_MAX_SCENE_SLOTS 64
+
 /* ---- context ------------------------------------------------------------ */
 /* Creates a context on HIP device `device`. Replaces the GL objects the
  * Renderer constructor allocates (Renderer.java:55-62). */
@@ -270,6 +273,34 @@ int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t);
  * host RGBA32F image (rows bottom-up) with N = 1, exposure 1 and no a-trous. */
 int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
                   const rc_temporal *t);
+
+/* ---- batched views (no counterpart in the reference) --------------------- */
+/* Serving many render requests at once: each view names a scene slot, a
+ * camera (with its own resolution), path-trace switches and a sample count.
+ * rc_views_render traces all views in ONE persistent launch; each view keeps
+ * its own accumulation and G-buffers, and its pixels are bit-identical to a
+ * single-view rc_render_passes of the same camera and samples. */
+typedef struct rc_view {
+    int32_t scene;          /* scene slot, 0..RC_MAX_SCENE_SLOTS-1            */
+    int32_t nSamples;       /* samples this call adds                         */
+    rc_camera cam;
+    rc_pt_params params;
+} rc_view;
+enum {
+    RC_VIEWS_RESET = 1,     /* restart every view's accumulation at frame 0   */
+    RC_VIEWS_FP64 = 2       /* FP64 accumulation (also kept in RGBA32F)       */
+};
+/* SceneUploader.upload into a scene slot (slot 0 = rc_upload_scene). */
+int rc_upload_scene_slot(rc_ctx *ctx, int32_t slot, const float *tris, int32_t nTris, const float *nodes,
+                         int32_t nNodes, const float *mats, int32_t nMats, const float *lights, int32_t nLights,
+                         float totalLightPower);
+/* A new set of views (different count or sizes) restarts accumulation. */
+int rc_views_render(rc_ctx *ctx, const rc_view *views, int32_t n, int32_t flags);
+/* The post chain (rc_post) on one view's accumulation and G-buffers. */
+int rc_views_post(rc_ctx *ctx, int32_t view, const rc_post_params *p, const rc_temporal *t);
+size_t rc_views_buffer_bytes(const rc_ctx *ctx, int32_t view, int32_t which);
+int rc_views_read(rc_ctx *ctx, int32_t view, int32_t which, void *dst, size_t bytes);
+int32_t rc_views_total_samples(const rc_ctx *ctx, int32_t view);
 
 /* ---- learned denoiser (no counterpart in the reference) ------------------ */
 /* A 5-level U-Net (3x3 convolutions, 32..128 channels) on bfloat16 MFMA with

@@ -84,6 +84,15 @@ class RcPostParams(C.Structure):
     ]
 
 
+class RcView(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("nSamples", C.c_int32), ("cam", RcCamera), ("params", RcPtParams)]
+
+
+RC_VIEWS_RESET = 1
+RC_VIEWS_FP64 = 2
+RC_MAX_SCENE_SLOTS = 64
+
+
 class RcTemporal(C.Structure):
     _fields_ = [("time", _F), ("agcGain", _F), ("frameIndex", _I)]
 
@@ -126,6 +135,13 @@ HIP_SYMBOLS = {
     "rc_post_defaults": (None, [C.POINTER(RcPostParams)]),
     "rc_post": (C.c_int, [_VP, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
     "rc_post_image": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
+    "rc_upload_scene_slot": (C.c_int, [_VP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP,
+                                       C.c_int32, C.c_float]),
+    "rc_views_render": (C.c_int, [_VP, C.POINTER(RcView), C.c_int32, C.c_int32]),
+    "rc_views_post": (C.c_int, [_VP, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
+    "rc_views_buffer_bytes": (C.c_size_t, [_VP, C.c_int32, C.c_int32]),
+    "rc_views_read": (C.c_int, [_VP, C.c_int32, C.c_int32, _VP, C.c_size_t]),
+    "rc_views_total_samples": (C.c_int32, [_VP, C.c_int32]),
     "rc_net_param_count": (C.c_int32, []),
     "rc_net_init": (C.c_int, [_VP, C.c_uint32]),
     "rc_net_set_params": (C.c_int, [_VP, _FP, C.c_int32]),

@@ -69,38 +69,56 @@ static int ensure_post(rc_ctx *ctx, int w, int h) {
 }
 
 // PostProcessStack.runChain (PostProcessStack.java:140-280) from the p00 input
-// `src` (accumulation sum, or an image with total = 1), then the display pass.
-static int run_post_chain(rc_ctx *ctx, const float4 *src, int w, int h, const rc_post_params &p,
-                          const rc_temporal &t, int total, bool withGbuf) {
-    Images &im = ctx->img;
-    float4 *a = im.postA, *b = im.postB;
-    hipError_t e = launch_post_pass(ctx->stream, 0, src, a, w, h, p, t, im.gbuf, im.galbedo, im.variance, total, 0,
-                                    nullptr);
+// io.src (accumulation sum, or an image with total = 1), then the display pass.
+int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf) {
+    const int w = io.w, h = io.h;
+    float4 *a = io.a, *b = io.b;
+    hipError_t e = launch_post_pass(ctx->stream, 0, io.src, a, w, h, p, t, io.gbuf, io.galbedo, io.variance,
+                                    io.total, 0, nullptr);
     if (e != hipSuccess) return hip_fail(ctx, e, "post p00 launch");
-    bool exceeded = p.atrousMaxSpp > 0 && total > p.atrousMaxSpp;
+    bool exceeded = p.atrousMaxSpp > 0 && io.total > p.atrousMaxSpp;
     if (p.atrousEnabled && withGbuf && !exceeded && ctx->denoiser == RC_DENOISE_NET && net_ready(ctx)) {
-        int rc = run_net_denoise(ctx, a, b, w, h);
+        int rc = run_net_denoise(ctx, a, io.gbuf, io.galbedo, b, w, h);
         if (rc) return rc;
         float4 *tmp = a; a = b; b = tmp;
     } else if (p.atrousEnabled && withGbuf && !exceeded) {
         int np = p.atrousIterations < 1 ? 1 : (p.atrousIterations > 5 ? 5 : p.atrousIterations);
         for (int i = 0; i < np; ++i) {  // AtrousDenoiser.denoise (AtrousDenoiser.java:60-125)
-            e = launch_post_pass(ctx->stream, 100, a, b, w, h, p, t, im.gbuf, im.galbedo, im.variance, total, 1 << i,
-                                 nullptr);
+            e = launch_post_pass(ctx->stream, 100, a, b, w, h, p, t, io.gbuf, io.galbedo, io.variance, io.total,
+                                 1 << i, nullptr);
             if (e != hipSuccess) return hip_fail(ctx, e, "post a-trous launch");
             float4 *tmp = a; a = b; b = tmp;
         }
     }
     for (int k = 1; k <= 16; ++k) {
         if (k == 5 || k == 6 || !p.enable[k]) continue;  // p05/p06 pass through (PostProcessStack.java:184)
-        e = launch_post_pass(ctx->stream, k, a, b, w, h, p, t, nullptr, nullptr, nullptr, total, 0, nullptr);
+        e = launch_post_pass(ctx->stream, k, a, b, w, h, p, t, nullptr, nullptr, nullptr, io.total, 0, nullptr);
         if (e != hipSuccess) return hip_fail(ctx, e, "post pass launch");
         float4 *tmp = a; a = b; b = tmp;
     }
-    e = launch_post_pass(ctx->stream, 200, a, nullptr, w, h, p, t, nullptr, nullptr, nullptr, 1, 0, im.display);
+    e = launch_post_pass(ctx->stream, 200, a, nullptr, w, h, p, t, nullptr, nullptr, nullptr, 1, 0, io.display);
     if (e != hipSuccess) return hip_fail(ctx, e, "display launch");
-    im.postOut = a;
+    io.out = a;
     return RC_OK;
+}
+
+static int run_post_chain(rc_ctx *ctx, const float4 *src, int w, int h, const rc_post_params &p,
+                          const rc_temporal &t, int total, bool withGbuf) {
+    Images &im = ctx->img;
+    ChainIO io;
+    io.src = src;
+    io.gbuf = im.gbuf;
+    io.galbedo = im.galbedo;
+    io.variance = im.variance;
+    io.w = w;
+    io.h = h;
+    io.total = total;
+    io.a = im.postA;
+    io.b = im.postB;
+    io.display = im.display;
+    int rc = run_chain(ctx, io, p, t, withGbuf);
+    if (rc == RC_OK) im.postOut = io.out;
+    return rc;
 }
 
 static int validate_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t) {
@@ -221,6 +239,8 @@ int rc_destroy(rc_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx->scene);
+    for (auto &d : ctx->extraScenes) free_scene(d);
+    free_views(ctx);
     free_images(ctx->img);
     free_sppm(ctx->sppm);
     free_wavefront(ctx->wf);
@@ -242,10 +262,11 @@ int rc_sync(rc_ctx *ctx) {
     return RC_OK;
 }
 
-int rc_upload_scene(rc_ctx *ctx, const float *tris, int32_t nTris, const float *nodes, int32_t nNodes,
-                    const float *mats, int32_t nMats, const float *lights, int32_t nLights,
-                    float totalLightPower) {
-    if (!ctx) return RC_ERR_INVALID_ARG;
+}  // extern "C"
+
+static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int32_t nTris, const float *nodes,
+                             int32_t nNodes, const float *mats, int32_t nMats, const float *lights, int32_t nLights,
+                             float totalLightPower) {
     if (nTris <= 0 || nNodes <= 0 || nMats <= 0 || nLights < 0)
         return set_error(ctx, RC_ERR_INVALID_ARG, "nTris, nNodes and nMats must be > 0, nLights >= 0");
     if (!tris || !nodes || !mats || (nLights > 0 && !lights))
@@ -338,8 +359,7 @@ int rc_upload_scene(rc_ctx *ctx, const float *tris, int32_t nTris, const float *
 
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
-    free_scene(ctx->scene);
-    DeviceScene &D = ctx->scene;
+    free_scene(D);
     int rc;
     if ((rc = dalloc(ctx, D.inner, inner.size() ? inner.size() : 4, "hipMalloc(nodes)"))) return rc;
     if ((rc = dalloc(ctx, D.triPos, pos.size(), "hipMalloc(triangles)"))) return rc;
@@ -365,6 +385,31 @@ int rc_upload_scene(rc_ctx *ctx, const float *tris, int32_t nTris, const float *
     D.maxDepth = maxDepth;
     D.valid = true;
     return RC_OK;
+}
+
+DeviceScene *rc::scene_slot(rc_ctx *ctx, int slot) {
+    if (slot == 0) return &ctx->scene;
+    if (slot >= 1 && slot < RC_MAX_SCENE_SLOTS) return &ctx->extraScenes[slot - 1];
+    return nullptr;
+}
+
+extern "C" {
+
+int rc_upload_scene(rc_ctx *ctx, const float *tris, int32_t nTris, const float *nodes, int32_t nNodes,
+                    const float *mats, int32_t nMats, const float *lights, int32_t nLights,
+                    float totalLightPower) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    return upload_scene_into(ctx, ctx->scene, tris, nTris, nodes, nNodes, mats, nMats, lights, nLights,
+                             totalLightPower);
+}
+
+int rc_upload_scene_slot(rc_ctx *ctx, int32_t slot, const float *tris, int32_t nTris, const float *nodes,
+                         int32_t nNodes, const float *mats, int32_t nMats, const float *lights, int32_t nLights,
+                         float totalLightPower) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    DeviceScene *D = scene_slot(ctx, slot);
+    if (!D) return set_error(ctx, RC_ERR_INVALID_ARG, "scene slot out of range");
+    return upload_scene_into(ctx, *D, tris, nTris, nodes, nNodes, mats, nMats, lights, nLights, totalLightPower);
 }
 
 static int render_common(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *p, int32_t n) {

@@ -91,7 +91,20 @@ struct Wavefront {
     int *hostCnt = nullptr;  // pinned
 };
 
-struct Net;  // learned denoiser (rc_net.hip)
+struct Net;    // learned denoiser (rc_net.hip)
+struct Views;  // batched views (rc_batch.hip)
+
+// Inputs and buffers of one post-chain run.
+struct ChainIO {
+    const float4 *src = nullptr;      // p00 input (accumulated sums)
+    const ushort4 *gbuf = nullptr;    // RGBA16F normal + depth (null: no G-buffer)
+    const ushort4 *galbedo = nullptr; // RGBA16F first-hit albedo
+    const float *variance = nullptr;  // R32F sum of luma^2
+    int w = 0, h = 0, total = 0;
+    float4 *a = nullptr, *b = nullptr;  // RGBA32F ping-pong
+    uchar4 *display = nullptr;          // RGBA8 output
+    const float4 *out = nullptr;        // set to whichever of a/b holds the chain output
+};
 
 }  // namespace rc
 
@@ -99,7 +112,9 @@ struct rc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    rc::DeviceScene scene;
+    rc::DeviceScene scene;                                // scene slot 0
+    rc::DeviceScene extraScenes[RC_MAX_SCENE_SLOTS - 1];  // slots 1..
+    rc::Views *views = nullptr;
     rc::Images img;
     int32_t totalSamples = 0;
     int32_t accumMode = RC_ACCUM_FP32;
@@ -131,7 +146,15 @@ void free_wavefront(Wavefront &w);
 void free_net(Net *n);
 bool net_ready(const rc_ctx *ctx);
 // Denoises the w x h RGBA32F post image src into dst with the frame's G-buffers.
-int run_net_denoise(rc_ctx *ctx, const float4 *src, float4 *dst, int w, int h);
+int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo, float4 *dst, int w,
+                    int h);
+
+// Post chain on explicit buffers (rc_capi.hip).
+int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf);
+// Scene of a slot (0 = rc_upload_scene's), or null when out of range.
+DeviceScene *scene_slot(rc_ctx *ctx, int slot);
+// Batched views (rc_batch.hip).
+void free_views(rc_ctx *ctx);
 
 // One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
 // with the given step, 200 = display encode into rgba8.

@@ -577,8 +577,8 @@ struct Net {
     // dataset
     float *raw = nullptr, *tgt = nullptr;
     int nCrops = 0, cropH = 0, cropW = 0;
-    // workspace for (N, H, W)
-    int wsN = 0, wsH = 0, wsW = 0;
+    // workspace: capacity in level-0 pixels (N*H*W); level k holds cap >> 2k
+    size_t wsCap = 0;
     bool wsTrain = false;
     uint16_t *feat = nullptr;
     uint16_t *act[kNumLayers] = {};   // e1..e5, d4..d1 (index 9 unused)
@@ -638,39 +638,47 @@ void free_ws(Net &n) {
     nfree(n.target);
     nfree(n.part);
     nfree(n.hpart);
-    n.wsN = n.wsH = n.wsW = 0;
+    n.wsCap = 0;
+    n.wsTrain = false;
     n.partBytes = 0;
 }
 
 size_t level_pix(int N, int H, int W, int level) { return (size_t)N * (H >> level) * (W >> level); }
+// Capacity of a level-k buffer for cap level-0 pixels (H, W multiples of 16).
+size_t level_cap(size_t cap, int level) { return cap >> (2 * level); }
 
 constexpr int kDwPixPerChunk = 1024;
 
+// Grows the workspace to hold N x H x W (H, W multiples of 16); buffers are
+// kept for smaller shapes, so alternating view sizes do not reallocate.
 hipError_t ensure_ws(rc_ctx *ctx, Net &n, int N, int H, int W, bool train) {
-    if (n.wsN == N && n.wsH == H && n.wsW == W && (n.wsTrain || !train)) return hipSuccess;
+    const size_t need = (size_t)N * H * W;
+    if (need <= n.wsCap && (n.wsTrain || !train)) return hipSuccess;
+    const size_t cap = need > n.wsCap ? need : n.wsCap;
+    const bool tr = train || n.wsTrain;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return e;
     free_ws(n);
     auto A = [&](auto *&p, size_t cnt) {
         if (e == hipSuccess) e = nalloc(p, cnt);
     };
-    A(n.feat, level_pix(N, H, W, 0) * kFeatC);
-    for (int L = 0; L < 9; ++L) A(n.act[L], level_pix(N, H, W, kLayers[L].level) * kLayers[L].cout);
-    for (int k = 0; k < 4; ++k) A(n.pooled[k], level_pix(N, H, W, k + 1) * kLayers[k].cout);
-    if (train) {
-        for (int L = 0; L < 9; ++L) A(n.dz[L], level_pix(N, H, W, kLayers[L].level) * kLayers[L].cout);
+    A(n.feat, cap * kFeatC);
+    for (int L = 0; L < 9; ++L) A(n.act[L], level_cap(cap, kLayers[L].level) * kLayers[L].cout);
+    for (int k = 0; k < 4; ++k) A(n.pooled[k], level_cap(cap, k + 1) * kLayers[k].cout);
+    if (tr) {
+        for (int L = 0; L < 9; ++L) A(n.dz[L], level_cap(cap, kLayers[L].level) * kLayers[L].cout);
         for (int j = 0; j < 4; ++j) {  // dxcat[j] for decoder d(j+1): layer 8 - j
             const LayerSpec &s = kLayers[8 - j];
-            A(n.dxcat[j], level_pix(N, H, W, s.level) * s.cinPad);
+            A(n.dxcat[j], level_cap(cap, s.level) * s.cinPad);
         }
         for (int k = 0; k < 4; ++k) {  // dxpool[k] for encoder e(k+2): level k+1, cin = cout of e(k+1)
-            A(n.dxpool[k], level_pix(N, H, W, k + 1) * kLayers[k].cout);
+            A(n.dxpool[k], level_cap(cap, k + 1) * kLayers[k].cout);
         }
-        A(n.target, level_pix(N, H, W, 0) * 3);
+        A(n.target, cap * 3);
         size_t maxPart = 0;
         for (int L = 0; L < 9; ++L) {
             const LayerSpec &s = kLayers[L];
-            size_t np = level_pix(N, H, W, s.level);
+            size_t np = level_cap(cap, s.level);
             size_t chunks = (np + kDwPixPerChunk - 1) / kDwPixPerChunk;
             size_t sz = chunks * (size_t)s.cout * 9 * s.cinPad;
             size_t csz = ((np + 255) / 256) * (size_t)s.cout;
@@ -679,16 +687,14 @@ hipError_t ensure_ws(rc_ctx *ctx, Net &n, int N, int H, int W, bool train) {
         }
         A(n.part, maxPart);
         n.partBytes = maxPart * 4;
-        A(n.hpart, ((level_pix(N, H, W, 0) + 255) / 256) * 100);
+        A(n.hpart, ((cap + 255) / 256) * 100);
     }
     if (e != hipSuccess) {
         free_ws(n);
         return e;
     }
-    n.wsN = N;
-    n.wsH = H;
-    n.wsW = W;
-    n.wsTrain = train;
+    n.wsCap = cap;
+    n.wsTrain = tr;
     return hipSuccess;
 }
 
@@ -945,13 +951,14 @@ bool net_ready(const rc_ctx *ctx) { return ctx->net && ctx->net->ready; }
 
 // Denoises the post image src (w x h RGBA32F, p00 output) into dst using the
 // frame's G-buffers.
-int run_net_denoise(rc_ctx *ctx, const float4 *src, float4 *dst, int w, int h) {
+int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo, float4 *dst, int w,
+                    int h) {
     Net &n = *ctx->net;
     const int Wp = (w + 15) / 16 * 16, Hp = (h + 15) / 16 * 16;
     hipError_t e = ensure_ws(ctx, n, 1, Hp, Wp, false);
     if (e != hipSuccess) return hip_fail(ctx, e, "net workspace");
-    hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)Wp * Hp)), dim3(256), 0, ctx->stream, src,
-                       ctx->img.gbuf, ctx->img.galbedo, w, h, Wp, Hp, n.feat);
+    hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)Wp * Hp)), dim3(256), 0, ctx->stream, src, gbuf,
+                       galbedo, w, h, Wp, Hp, n.feat);
     if ((e = forward(ctx, n, 1, Hp, Wp)) != hipSuccess) return hip_fail(ctx, e, "net forward");
     HeadArgs hd{};
     hd.d1 = n.act[8];

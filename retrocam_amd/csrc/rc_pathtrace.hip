@@ -22,33 +22,9 @@
 
 namespace rc {
 
-struct PassArgs {
-    SceneView S;
-    CamArgs cam;
-    int nee;
-    float neeClamp;
-    int maxBounces;
-    int frame0;
-    int nSamples;
-    float4 *accum;
-    double4 *accum64;
-    ushort4 *gbuf;
-    ushort4 *galbedo;
-    float *variance;
-    int *primId;
-    int *gbufId;
-    unsigned long long *counters;
-};
-
+// All samples of one pixel (px, py) of the pass described by A.
 template <bool STATS, bool FP64>
-__global__ void __launch_bounds__(kBlock) pathtrace_kernel(PassArgs A) {
-    __shared__ int2 stk[kShortStack][kBlock];
-    const StackRef st{stk, (int)threadIdx.x};
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    Counters C;
-    if (STATS) C = Counters{};
+__device__ __forceinline__ void trace_pixel(const PassArgs &A, int px, int py, const StackRef &st, Counters &C) {
     const SceneView &S = A.S;
     const CamArgs &cam = A.cam;
     if (px < cam.width && py < cam.height) {
@@ -183,8 +159,59 @@ __global__ void __launch_bounds__(kBlock) pathtrace_kernel(PassArgs A) {
         A.variance[pi] = var;
         A.primId[pi] = lastTri;
     }
+}
+
+template <bool STATS, bool FP64>
+__global__ void __launch_bounds__(kBlock) pathtrace_kernel(PassArgs A) {
+    __shared__ int2 stk[kShortStack][kBlock];
+    const StackRef st{stk, (int)threadIdx.x};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    Counters C;
+    if (STATS) C = Counters{};
+    trace_pixel<STATS, FP64>(A, px, py, st, C);
     if (STATS) wave_add_counters(A.counters, C);
 }
+
+// Many views (own scene, camera, sample count and images) in one persistent
+// launch: each workgroup takes 16x16 tiles from a global counter until all
+// tiles of all views are done. tileOff[v] is the first tile of view v (views
+// are laid out heaviest first, so long tiles start early).
+template <bool STATS, bool FP64>
+__global__ void __launch_bounds__(kBlock) pathtrace_views_kernel(const PassArgs *views, const int *tileOff,
+                                                                 int nViews, int *counter,
+                                                                 unsigned long long *counters) {
+    __shared__ int2 stk[kShortStack][kBlock];
+    __shared__ int tileSh;
+    const StackRef st{stk, (int)threadIdx.x};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int totalTiles = tileOff[nViews];
+    Counters C;
+    if (STATS) C = Counters{};
+    while (true) {
+        if (threadIdx.x == 0) tileSh = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = tileSh;
+        __syncthreads();
+        if (t >= totalTiles) break;
+        int lo = 0, hi = nViews;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (tileOff[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const PassArgs A = views[lo];
+        const int lt = t - tileOff[lo];
+        const int tilesX = (A.cam.width + 15) >> 4;
+        const int px = (lt % tilesX) * 16 + (wave & 1) * 8 + (lane & 7);
+        const int py = (lt / tilesX) * 16 + (wave >> 1) * 8 + (lane >> 3);
+        trace_pixel<STATS, FP64>(A, px, py, st, C);
+    }
+    if (STATS) wave_add_counters(counters, C);
+}
+
+__global__ void zero_counter(int *c) { *c = 0; }
 
 CamArgs cam_args(const rc_camera &cam) {
     CamArgs c;
@@ -227,6 +254,30 @@ hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_param
     else if (stats) hipLaunchKernelGGL((pathtrace_kernel<true, false>), grid, dim3(kBlock), 0, ctx->stream, A);
     else if (fp64) hipLaunchKernelGGL((pathtrace_kernel<false, true>), grid, dim3(kBlock), 0, ctx->stream, A);
     else hipLaunchKernelGGL((pathtrace_kernel<false, false>), grid, dim3(kBlock), 0, ctx->stream, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_pathtrace_views(rc_ctx *ctx, const PassArgs *views, const int *tileOff, int nViews, int *counter,
+                                  int maxBlocks, bool fp64) {
+    const bool stats = ctx->countersEnabled && ctx->counters;
+    static int occ[4] = {0, 0, 0, 0};
+    const int variant = (stats ? 2 : 0) + (fp64 ? 1 : 0);
+    auto kernel = stats ? (fp64 ? pathtrace_views_kernel<true, true> : pathtrace_views_kernel<true, false>)
+                        : (fp64 ? pathtrace_views_kernel<false, true> : pathtrace_views_kernel<false, false>);
+    if (occ[variant] == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, 0) != hipSuccess || nb < 1) nb = 1;
+        occ[variant] = nb;
+    }
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    int grid = occ[variant] * cus;
+    if (grid > maxBlocks) grid = maxBlocks;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, ctx->stream, counter);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, views, tileOff, nViews, counter,
+                       ctx->counters);
     return hipGetLastError();
 }
 

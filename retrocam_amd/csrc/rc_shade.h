@@ -5,6 +5,7 @@
 #pragma once
 
 #include "rc_device.h"
+#include "retrocam_hip.h"
 
 namespace rc {
 
@@ -13,6 +14,25 @@ struct CamArgs {
     float focalDist, lensRadius, tanHalf, saStrength;
     float lca[3];
     int blades, width, height;
+};
+
+// One progressive launch: nSamples passes from frame0 on, over the images given.
+struct PassArgs {
+    SceneView S;
+    CamArgs cam;
+    int nee;
+    float neeClamp;
+    int maxBounces;
+    int frame0;
+    int nSamples;
+    float4 *accum;
+    double4 *accum64;
+    ushort4 *gbuf;
+    ushort4 *galbedo;
+    float *variance;
+    int *primId;
+    int *gbufId;
+    unsigned long long *counters;
 };
 
 __device__ __forceinline__ v3 sample_vndf(v3 wo, float alpha, float ux, float uy) {
@@ -242,5 +262,11 @@ __device__ __forceinline__ bool shade_vertex(const SceneView &S, int neeEnabled,
     }
     return false;
 }
+
+CamArgs cam_args(const rc_camera &cam);  // rc_pathtrace.hip
+// Persistent launch over nViews views (rc_pathtrace.hip); views and tileOff in
+// device memory, counter a device int the launch resets first.
+hipError_t launch_pathtrace_views(rc_ctx *ctx, const PassArgs *views, const int *tileOff, int nViews, int *counter,
+                                  int maxBlocks, bool fp64);
 
 }  // namespace rc

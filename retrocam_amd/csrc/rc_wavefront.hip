@@ -23,8 +23,6 @@
 
 namespace rc {
 
-CamArgs cam_args(const rc_camera &cam);  // rc_pathtrace.hip
-
 namespace {
 
 // flag bits of the slot state (bounce counts up to maxBounces = 16)

@@ -133,17 +133,47 @@ def render_pair(ctx, w, spp_in: int, spp_ref: int):
     return raw, tgt
 
 
-def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32), spp_ref: int = 4096, log=None):
-    """Pairs over workloads.training_scene(seed); spp_in cycles through the given counts."""
+def _raw_from(acc, gbuf, galb, w, h):
+    gb = _half(gbuf)
+    ga = _half(galb)
+    raw = np.zeros((h * w, 10), np.float32)
+    raw[:, 0:3] = acc[:, 0:3] / acc[:, 3:4]
+    raw[:, 3:6] = ga[:, 0:3]
+    raw[:, 6:9] = gb[:, 0:3]
+    raw[:, 9] = gb[:, 3]
+    return raw.reshape(h, w, 10)
+
+
+def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32), spp_ref: int = 4096, log=None,
+                 group: int = N.RC_MAX_SCENE_SLOTS):
+    """Pairs over workloads.training_scene(seed); spp_in cycles through the given
+    counts. Views are rendered in batches of `group` scenes, one persistent
+    launch per batch and sample chunk (rc_views_render)."""
     from . import workloads
+    from .batch import ViewBatch, ViewRequest
+    seeds = list(seeds)
+    vb = ViewBatch(ctx)
     raws, tgts = [], []
-    for i, s in enumerate(seeds):
-        w = workloads.training_scene(int(s), size=size)
-        raw, tgt = render_pair(ctx, w, int(spp_in[i % len(spp_in)]), spp_ref)
-        raws.append(raw)
-        tgts.append(tgt)
-        if log and (i + 1) % 10 == 0:
-            log(f"[dataset] {i + 1}/{len(seeds)} scenes")
+    for g0 in range(0, len(seeds), group):
+        ws = [workloads.training_scene(int(s), size=size) for s in seeds[g0:g0 + group]]
+        for k, w in enumerate(ws):
+            vb.upload_scene(k, w.scene)
+        reqs = [ViewRequest(k, w.camera, w.params, int(spp_in[(g0 + k) % len(spp_in)])) for k, w in enumerate(ws)]
+        vb.render(reqs, reset=True)
+        for k in range(len(ws)):
+            raws.append(_raw_from(vb.read(k, N.RC_BUF_ACCUM), vb.read(k, N.RC_BUF_GBUFFER),
+                                  vb.read(k, N.RC_BUF_GALBEDO), size, size))
+        done = 0
+        while done < spp_ref:
+            kk = min(512, spp_ref - done)
+            vb.render([ViewRequest(q.scene, q.camera, q.params, kk) for q in reqs], reset=done == 0, fp64=True)
+            done += kk
+            ctx.sync()
+        for k in range(len(ws)):
+            acc = vb.read(k, N.RC_BUF_ACCUM64)
+            tgts.append((acc[:, 0:3] / acc[:, 3:4]).astype(np.float32).reshape(size, size, 3))
+        if log:
+            log(f"[dataset] {min(g0 + group, len(seeds))}/{len(seeds)} scenes")
     return np.stack(raws), np.stack(tgts)
 
 

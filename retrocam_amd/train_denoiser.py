@@ -1,0 +1,110 @@
+"""Trains the learned denoiser on the GPU and writes its weights.
+
+    python -m retrocam_amd.train_denoiser [--scenes 200] [--size 128] [--ref-spp 4096]
+        [--steps 4000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
+
+Training data: seeded views from workloads.training_scene rendered by this
+package's path tracer (inputs at 4/8/16/32 spp cycling over the scenes, targets
+at --ref-spp with FP64 accumulation). The optimizer runs entirely in
+libretrocam_hip.so (rc_net_train: forward, backward and Adam kernels).
+Evaluation: relMSE on held-out views (seeds >= 1,000,000) against their
+high-spp renders, for the noisy input, a-trous and the network.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _native as N
+from . import workloads
+from .denoiser import WEIGHTS_PATH, NetDenoiser, make_dataset, rel_mse, render_pair
+from .post import PostProcessStack, TemporalInputs, make_params
+from .renderer import RenderContext
+
+
+def atrous_of(ctx):
+    """a-trous (5 passes, default settings) of the frame in ctx's accumulation."""
+    post = PostProcessStack(ctx)
+    return post.run_on_accum(make_params(atrous=True, iterations=5), TemporalInputs()).hdr[:, :3]
+
+
+def evaluate(ctx, net, seeds, size, spp_in, ref_spp):
+    rows = []
+    for s in seeds:
+        w = workloads.training_scene(int(s), size=size)
+        raw, tgt = render_pair(ctx, w, spp_in, ref_spp)
+        # à-trous on the same low-spp frame (render_pair leaves the reference in the buffers)
+        from .renderer import Renderer
+        r = Renderer(ctx)
+        r.reset()
+        r.render(w.camera, w.params, n_samples=spp_in)
+        at = atrous_of(ctx).reshape(size, size, 3)
+        out = net.forward(raw[None])[0]
+        rows.append((rel_mse(raw[..., 0:3], tgt), rel_mse(at, tgt), rel_mse(out, tgt)))
+    return np.array(rows)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scenes", type=int, default=200)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--eval-scenes", type=int, default=8)
+    ap.add_argument("--out", default=WEIGHTS_PATH)
+    ap.add_argument("--summary", default=None, help="write a JSON summary here")
+    args = ap.parse_args(argv)
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    ctx = RenderContext(0)
+    t0 = time.perf_counter()
+    raw, tgt = make_dataset(ctx, range(args.scenes), size=args.size, spp_ref=args.ref_spp, log=log)
+    t_data = time.perf_counter() - t0
+    log(f"[train] dataset: {raw.shape[0]} views {args.size}x{args.size} in {t_data:.1f} s")
+    net = NetDenoiser(ctx).init(args.seed)
+    net.set_dataset(raw, tgt)
+    t1 = time.perf_counter()
+    chunk = 250
+    losses = []
+    done = 0
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        frac = done / args.steps
+        lr = args.lr * (1.0 if frac < 0.6 else (0.3 if frac < 0.85 else 0.1))
+        losses.append(net.train(k, batch=args.batch, lr=lr, seed=args.seed * 7919 + done))
+        done += k
+        log(f"[train] step {done}/{args.steps} lr {lr:.1e} loss {float(np.mean(losses[-1][-50:])):.5f}")
+    ctx.sync()
+    t_train = time.perf_counter() - t1
+    losses = np.concatenate(losses)
+    net.save(args.out)
+    log(f"[train] {args.steps} steps in {t_train:.1f} s; weights -> {args.out}")
+    ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.ref_spp)
+    summary = {
+        "scenes": args.scenes, "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
+        "batch": args.batch, "lr": args.lr, "params": net.n,
+        "dataset_s": round(t_data, 2), "train_s": round(t_train, 2),
+        "ms_per_step": round(t_train / args.steps * 1e3, 3),
+        "loss_first50": float(np.mean(losses[:50])), "loss_last50": float(np.mean(losses[-50:])),
+        "heldout_8spp_relmse": {"input": float(ev[:, 0].mean()), "atrous": float(ev[:, 1].mean()),
+                                "net": float(ev[:, 2].mean())},
+    }
+    log(f"[train] held-out relMSE at 8 spp: {summary['heldout_8spp_relmse']}")
+    if args.summary:
+        with open(args.summary, "w") as f:
+            json.dump(summary, f, indent=1)
+    print(json.dumps(summary))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -144,13 +144,15 @@ def sppm_caustics(seed: int = 1, width: int = 854, height: int = 480, spp: int =
 
 def training_scene(seed: int, size: int = 128, spp: int = 8) -> Workload:
     """A seeded denoiser-training view (size x size): a closed room with random
-    wall albedos, 3-10 jittered spheres/boxes with the full-frame material mix
-    (50% diffuse, 30% GGX metal, 20% glass), 1-3 quad lights of random strength
-    and a random thin-lens or pinhole camera looking into the room."""
+    wall albedos, 3-10 large (even seeds) or 40-160 small (odd seeds) jittered
+    spheres/boxes with the full-frame material mix (50% diffuse, 30% GGX metal,
+    20% glass), 1-3 quad lights of random strength and a random thin-lens or
+    pinhole camera looking into the room."""
     rng = np.random.default_rng(100_000 + seed)
     b = SceneBuilder()
     walls = [b.add_material(Material.diffuse(*rng.uniform(0.1, 0.8, 3))) for _ in range(6)]
     X, Y, Z = float(rng.uniform(4, 8)), float(rng.uniform(4, 7)), float(rng.uniform(4, 8))
+    clutter = seed % 2 == 1  # odd seeds: many small objects, as in the full frame
     b.add_quad((-X, 0, -Z), (-X, 0, Z), (X, 0, Z), (X, 0, -Z), walls[0])
     b.add_quad((-X, Y, -Z), (X, Y, -Z), (X, Y, Z), (-X, Y, Z), walls[1])
     b.add_quad((-X, 0, -Z), (X, 0, -Z), (X, Y, -Z), (-X, Y, -Z), walls[2])
@@ -172,9 +174,9 @@ def training_scene(seed: int, size: int = 128, spp: int = 8) -> Workload:
             return b.add_material(Material.metal(*rng.uniform(0.5, 0.95, 3), float(rng.uniform(0.05, 0.6))))
         return b.add_material(Material.glass(1.5))
 
-    for _ in range(int(rng.integers(3, 11))):
+    for _ in range(int(rng.integers(40, 160)) if clutter else int(rng.integers(3, 11))):
         x, z = rng.uniform(-0.8 * X, 0.8 * X), rng.uniform(-0.8 * Z, 0.8 * Z)
-        r = float(rng.uniform(0.3, 1.2))
+        r = float(rng.uniform(0.15, 0.6)) if clutter else float(rng.uniform(0.3, 1.2))
         y = r + float(rng.uniform(0.0, 1.5)) * (rng.random() < 0.3)
         if rng.random() < 0.5:
             b.add_sphere((x, y, z), r, material(), stacks=16, slices=32)
