@@ -1,17 +1,21 @@
 """Benchmark of the render hot path on one MI355X.
 
-Workload (default): the full plain frame — 854x480, 128 spp per frame, 6
-bounces, NEE, thin lens 35 mm f/2.8 with hexagon bokeh, over a seeded
-250,000-triangle scene (retrocam_amd.workloads.full_frame), then the post
-chain: p00 normalize, 5 a-trous passes, all 14 VHS passes and the display
-encode. One step = one frame: reset, 128 progressive samples in one launch of
-the path-trace kernel (the same arithmetic as 128 Renderer.render passes),
-then the post chain (RenderPipeline.accumulateFrame without SPPM,
+Workload: the full plain frame — 854x480, 6 bounces, NEE, thin lens 35 mm
+f/2.8 with hexagon bokeh, over a seeded 250,000-triangle scene
+(retrocam_amd.workloads.full_frame), then the post chain: p00 normalize, the
+denoiser, all 14 VHS passes and the display encode. One frame = reset, spp
+progressive samples in one launch of the path-trace kernel (the same
+arithmetic as spp Renderer.render passes), then the post chain
+(RenderPipeline.accumulateFrame without SPPM,
 src/main/java/com/retrocam/export/RenderPipeline.java:209-239).
 
 Quality: relMSE = mean over pixels and RGB of (x - r)^2 / (r^2 + 0.01) on linear
-HDR after p00 (+ a-trous), against r = a 4096-spp FP64-accumulated render of
-the same frame by the same path tracer.
+HDR after p00 + denoiser, against r = a 4096-spp FP64-accumulated render of the
+same frame by the same path tracer. For each method (no denoiser, a-trous x5,
+the learned U-Net) the smallest spp of the sweep with relMSE <= 0.01 is its
+operating point. The timed steps are frames at the U-Net's operating point
+(the headline); the plain 128-spp a-trous frame and the other methods'
+operating points are timed beside it.
 
 Prints ONE JSON line.
 """
@@ -87,9 +91,10 @@ class HipTimer:
         return out
 
 
-def cpu_baseline(w, spp_sample: int):
+def cpu_baseline(w, spp_sample: int, spp_frame: int):
     """The oracle (scalar C restatement, OpenMP over rows) on spp_sample samples
-    of the same frame; returns frames/s scaled per spp and Mrays/s."""
+    of the same frame; returns frames/s of a spp_frame-sample frame (scaled per
+    spp) and Mrays/s."""
     from oracle import rco
     o = rco.OracleRenderer(w.scene, w.camera.to_c(), w.params.to_c())
     t0 = time.perf_counter()
@@ -98,9 +103,9 @@ def cpu_baseline(w, spp_sample: int):
     dt = time.perf_counter() - t0
     st = o.stats.as_dict()
     rays = st["rays"] + st["shadowRays"]
-    fps = 1.0 / (dt / spp_sample * w.spp)
+    fps = 1.0 / (dt / spp_sample * spp_frame)
     return {"value": fps, "unit": "frames/s", "cores": int(rco.lib().rco_num_threads()), "kind": "port",
-            "sample": f"{spp_sample} of {w.spp} spp of the same {w.camera.width}x{w.camera.height} frame "
+            "sample": f"{spp_sample} of {spp_frame} spp of the same {w.camera.width}x{w.camera.height} frame "
                       f"({dt:.1f} s), scaled per spp",
             "mrays_per_s": rays / dt / 1e6}
 
@@ -122,15 +127,33 @@ def rel_mse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
 
 
+def read_traffic():
+    """HBM bytes per path-trace launch of 128 spp from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "pathtrace_kernel__full_frame__pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def rel_mse(x, r):
+    return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=128)
+    ap.add_argument("--fixed-spp", type=int, default=0,
+                    help="time frames at this spp with a-trous (skips the quality search when --ref-spp 0)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
-    ap.add_argument("--sweep", default="16,32,64,128,256,512",
-                    help="spp values searched for the relMSE <= 0.01 operating point ('' = skip)")
+    ap.add_argument("--sweep", default="4,8,16,32,64,128,256,512,1024",
+                    help="spp values searched for the relMSE <= 0.01 operating points")
     ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["wavefront", "megakernel"], default="megakernel")
     ap.add_argument("--copy-calib", action="store_true",
@@ -146,6 +169,7 @@ def main():
 
     from retrocam_amd import _native as N
     from retrocam_amd import workloads
+    from retrocam_amd.denoiser import NetDenoiser
     from retrocam_amd.post import VHS_PASSES, PostProcessStack, TemporalInputs, make_params
     from retrocam_amd.renderer import COUNTER_FIELDS, RenderContext, Renderer, SceneUploader
 
@@ -153,13 +177,14 @@ def main():
         if not args.quiet:
             print(*a, file=sys.stderr, flush=True)
 
-    w = workloads.full_frame(spp=args.spp)
+    w = workloads.full_frame(spp=128)
     log(f"[bench] scene: {w.scene.nTris} tris, {w.scene.nNodes} nodes, {w.scene.nLights} lights, "
         f"BVH depth {w.scene.maxDepth}")
     ctx = RenderContext(0)
     SceneUploader(ctx).upload(w.scene)
     r = Renderer(ctx)
     r.set_pt_mode(N.RC_PT_WAVEFRONT if args.pt_mode == "wavefront" else N.RC_PT_MEGAKERNEL)
+    net = NetDenoiser(ctx).load()
     post = PostProcessStack(ctx)
     pp_full = make_params(enable=VHS_PASSES, atrous=True, iterations=5)
     pp_denoise = make_params(atrous=True, iterations=5)
@@ -171,58 +196,29 @@ def main():
         r.reset()
         r.render(w.camera, w.params, n_samples=spp)
 
-    # Counting pass (separately compiled instrumented kernel, identical results):
-    r.debug_counters(True)
-    trace(w.spp)
-    ctx.sync()
-    cnt = dict(zip(COUNTER_FIELDS, (int(v) for v in r.debug_counters(False))))
-    alg_bytes = sum(BYTE_MODEL[k] * cnt[k] for k in BYTE_MODEL) + PIXEL_BYTES_PER_SPP * npx * w.spp
-    rays = cnt["rays"] + cnt["shadowRays"]
-    log(f"[bench] counters per frame: {cnt}")
+    def denoised(method):
+        net.enable(method == "net")
+        post.run_on_accum(pp_raw if method == "none" else pp_denoise, temporal, read=False)
+        return r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
 
-    for _ in range(args.warmup):
-        trace(w.spp)
-        post.run_on_accum(pp_full, temporal, read=False)
-    ctx.sync()
-    torch.cuda.synchronize()
-    t_trace = HipTimer(ctx.stream)
-    t_post = HipTimer(ctx.stream)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t_trace.start()
-        trace(w.spp)
-        t_trace.stop()
-        t_post.start()
-        post.run_on_accum(pp_full, temporal, read=False)
-        t_post.stop()
-    ctx.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kms = t_trace.times_ms()
-    pms = t_post.times_ms()
-    ms_per_step = wall / args.steps * 1e3
-    fps = args.steps / wall
-    avg_kernel_s = float(np.mean(kms)) / 1e3
-    achieved = alg_bytes / avg_kernel_s / 1e9
-    post_ms = float(np.mean(pms))
-    log(f"[bench] per-frame trace ms: {[round(x, 2) for x in kms]}; post chain ms: {[round(x, 3) for x in pms]}")
+    def frame(spp, method):
+        trace(spp)
+        net.enable(method == "net")
+        p = make_params(enable=VHS_PASSES, atrous=method != "none", iterations=5)
+        post.run_on_accum(p, temporal, read=False)
 
-    if args.copy_calib:
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        a.fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("[bench] copy calibration: 3 x 1 GiB device-to-device copies done")
+    def time_frames(spp, method, n):
+        t = HipTimer(ctx.stream)
+        for _ in range(n):
+            t.start()
+            frame(spp, method)
+            t.stop()
+        return float(np.mean(t.times_ms()))
 
+    methods = ["none", "atrous", "net"]
     quality = None
+    op = {}
     if args.ref_spp > 0:
-        # the 128-spp frame of the timed loop is still in the accumulation buffer
-        post.run_on_accum(pp_raw, temporal, read=False)
-        raw = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
-        post.run_on_accum(pp_denoise, temporal, read=False)
-        den = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
         r.set_accum_mode(N.RC_ACCUM_FP64)
         r.reset()
         done = 0
@@ -236,34 +232,87 @@ def main():
         acc64 = r.read(N.RC_BUF_ACCUM64)
         ref = acc64[:, :3] / acc64[:, 3:4] * float(pp_full.exposure)
         r.set_accum_mode(N.RC_ACCUM_FP32)
-        quality = {"reference_spp": args.ref_spp, "relmse_raw": rel_mse(raw, ref), "relmse_atrous": rel_mse(den, ref),
-                   "target": 0.01}
-        sweep = [int(v) for v in args.sweep.split(",") if v.strip()]
-        curve = []
-        for spp in sweep:
+        curves = {m: [] for m in methods}
+        for spp in [int(v) for v in args.sweep.split(",") if v.strip()]:
             trace(spp)
-            post.run_on_accum(pp_denoise, temporal, read=False)
-            d = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
-            curve.append([spp, round(rel_mse(d, ref), 6)])
-        quality["atrous_curve"] = curve
-        ok = [s for s, e in curve if e <= 0.01]
-        if ok:
-            s_star = min(ok)
-            tq = HipTimer(ctx.stream)
-            for _ in range(3):
-                tq.start()
-                trace(s_star)
-                post.run_on_accum(pp_full, temporal, read=False)
-                tq.stop()
-            quality["operating_point"] = {"spp": s_star, "frames_per_s": round(1e3 / float(np.mean(tq.times_ms())), 4),
-                                          "method": "path tracer + a-trous x5 + VHS"}
-        log(f"[bench] quality: {quality}")
+            for m in methods:
+                curves[m].append([spp, round(rel_mse(denoised(m), ref), 6)])
+            log(f"[bench] spp {spp}: " + ", ".join(f"{m} {curves[m][-1][1]}" for m in methods))
+        quality = {"reference_spp": args.ref_spp, "target": 0.01, "curves": curves}
+        for m in methods:
+            ok = [s for s, e in curves[m] if e <= 0.01]
+            if ok:
+                s_m = min(ok)
+                ms = time_frames(s_m, m, 3)
+                op[m] = {"spp": s_m, "relmse": dict(curves[m])[s_m], "frames_per_s": round(1e3 / ms, 4)}
+        quality["operating_points"] = op
+        if "net" in op:
+            base = op.get("atrous") or op.get("none")
+            if base:
+                quality["net_speedup_vs_plain"] = round(op["net"]["frames_per_s"] / base["frames_per_s"], 2)
+                quality["plain_method"] = "atrous" if "atrous" in op else "none (a-trous never reaches 0.01)"
+        log(f"[bench] operating points: {op}")
+
+    if "net" in op and not args.fixed_spp:
+        spp, method = op["net"]["spp"], "net"
+    else:
+        spp, method = (args.fixed_spp or 128), "atrous"
+    quality_met = bool(quality and method in op and op[method]["spp"] == spp)
+
+    # Counting pass (separately compiled instrumented kernel, identical results):
+    r.debug_counters(True)
+    trace(spp)
+    ctx.sync()
+    cnt = dict(zip(COUNTER_FIELDS, (int(v) for v in r.debug_counters(False))))
+    alg_bytes = sum(BYTE_MODEL[k] * cnt[k] for k in BYTE_MODEL) + PIXEL_BYTES_PER_SPP * npx * spp
+    rays = cnt["rays"] + cnt["shadowRays"]
+    log(f"[bench] counters per frame at {spp} spp: {cnt}")
+
+    for _ in range(args.warmup):
+        frame(spp, method)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t_trace = HipTimer(ctx.stream)
+    t_post = HipTimer(ctx.stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t_trace.start()
+        trace(spp)
+        t_trace.stop()
+        t_post.start()
+        net.enable(method == "net")
+        post.run_on_accum(make_params(enable=VHS_PASSES, atrous=True, iterations=5), temporal, read=False)
+        t_post.stop()
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = t_trace.times_ms()
+    pms = t_post.times_ms()
+    ms_per_step = wall / args.steps * 1e3
+    fps = args.steps / wall
+    avg_kernel_s = float(np.mean(kms)) / 1e3
+    achieved = alg_bytes / avg_kernel_s / 1e9
+    post_ms = float(np.mean(pms))
+    log(f"[bench] per-frame trace ms: {[round(x, 2) for x in kms]}; post chain ms: {[round(x, 3) for x in pms]}")
+
+    plain128 = round(1e3 / time_frames(128, "atrous", 3), 4)
+    net.enable(False)
+
+    if args.copy_calib:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 3 x 1 GiB device-to-device copies done")
 
     cpu = None
     if args.cpu_spp > 0:
         log(f"[bench] CPU baseline: oracle, {args.cpu_spp} spp ...")
-        cpu = cpu_baseline(w, args.cpu_spp)
+        cpu = cpu_baseline(w, args.cpu_spp, spp)
 
+    traffic128 = read_traffic()
     out = {
         "metric": "854x480 frames/s at relMSE<=0.01 vs 4096spp (net vs a-trous); Mrays/s",
         "value": round(fps, 4),
@@ -276,20 +325,24 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded 250k-triangle scene, generated in-process)",
-        "config": {"workload": "full plain frame: 250k tris, 854x480, 128 spp, 6 bounces, a-trous x5, 14 VHS passes",
+        "data": "synthetic (seeded 250k-triangle scene, generated in-process; denoiser weights trained on "
+                "seeded views by retrocam_amd.train_denoiser)",
+        "config": {"workload": f"full plain frame: 250k tris, 854x480, {spp} spp, 6 bounces, "
+                               f"{'U-Net' if method == 'net' else 'a-trous x5'} denoiser, 14 VHS passes",
                    "width": w.camera.width, "height": w.camera.height,
-                   "spp": w.spp, "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
+                   "spp": spp, "denoiser": method, "quality_met": quality_met,
+                   "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
                    "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU",
                    "pt_mode": args.pt_mode},
         "mrays_per_s": round(rays * 1e3 / float(np.mean(kms)) / 1e6, 2),
         "rays_per_frame": rays,
         "trace_ms": round(float(np.mean(kms)), 3),
         "post_ms": round(post_ms, 4),
-        "post_gbs_compulsory": round(POST_COMPULSORY_BYTES * npx / (post_ms / 1e3) / 1e9, 1),
+        "plain_128spp_atrous_frames_per_s": plain128,
         "quality": quality,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": int(traffic128 * spp / 128) if traffic128 else None,
                      "kernel": "pathtrace_kernel" if args.pt_mode == "megakernel" else "wavefront frame",
                      "alg_bytes_per_launch": int(alg_bytes),
                      "avg_launch_ms": round(avg_kernel_s * 1e3, 3)},
