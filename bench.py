@@ -152,7 +152,7 @@ def main():
     ap.add_argument("--fixed-spp", type=int, default=0,
                     help="time frames at this spp with a-trous (skips the quality search when --ref-spp 0)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
-    ap.add_argument("--sweep", default="4,8,16,32,64,128,256,512,1024",
+    ap.add_argument("--sweep", default="4,8,16,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
                     help="spp values searched for the relMSE <= 0.01 operating points")
     ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["wavefront", "megakernel"], default="megakernel")

@@ -305,8 +305,9 @@ int32_t rc_views_total_samples(const rc_ctx *ctx, int32_t view);
 /* ---- learned denoiser (no counterpart in the reference) ------------------ */
 /* A 5-level U-Net (3x3 convolutions, 32..128 channels) on bfloat16 MFMA with
  * FP32 accumulation, trained on the GPU by its own forward/backward kernels.
- * Inputs per pixel ("raw", 10 floats): radiance rgb, albedo rgb, normal xyz,
- * depth — the render outputs of pathtrace.comp:6-7, 644-647. Targets: 3 floats
+ * Inputs per pixel ("raw", 12 floats): radiance rgb, albedo rgb, normal xyz,
+ * depth, mean of luma^2 over the samples, sample count — the render outputs of
+ * pathtrace.comp:6-7, 644-654 (accumulation, G-buffers, variance). Targets: 3 floats
  * (high-spp radiance). Images are n x h x w, h and w multiples of 16. The flat
  * parameter vector holds, per layer, W[cout][ky][kx][cin] then b[cout]. */
 int32_t rc_net_param_count(void);
@@ -314,7 +315,7 @@ int32_t rc_net_param_count(void);
 int rc_net_init(rc_ctx *ctx, uint32_t seed);
 int rc_net_set_params(rc_ctx *ctx, const float *params, int32_t count);
 int rc_net_get_params(rc_ctx *ctx, float *params, int32_t count);
-/* Copies a training set to the device: raw n x h x w x 10, target n x h x w x 3. */
+/* Copies a training set to the device: raw n x h x w x 12, target n x h x w x 3. */
 int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32_t n, int32_t h, int32_t w);
 /* `steps` Adam steps (b1 0.9, b2 0.999) on batches of `batch` crops drawn
  * from the dataset with random flips; loss = mean (x-r)^2/(r^2+0.01). Writes

@@ -7,7 +7,7 @@ TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP forward and backward
 kernels of libretrocam_hip.so. Nothing in the product imports it.
 
 Network (5-level U-Net, 3x3 convolutions, zero 'same' padding, ReLU):
-  e1 = relu(conv(f))                      10 -> 32   level 0
+  e1 = relu(conv(f))                      12 -> 32   level 0
   e2 = relu(conv(pool(e1)))               32 -> 64   level 1
   e3 = relu(conv(pool(e2)))               64 -> 64   level 2
   e4 = relu(conv(pool(e3)))               64 -> 96   level 3
@@ -18,12 +18,16 @@ Network (5-level U-Net, 3x3 convolutions, zero 'same' padding, ReLU):
   d1 = relu(conv([up(d2), e1]))           64 -> 32   level 0
   o  = conv1x1(d1)                         32 -> 3
   y  = o + f[..., 0:3]    (residual in the log1p domain)
-  x  = expm1(clip(y, 0, 20))   (linear radiance)
+  x  = m * expm1(min(y, 20)),  m = max(f[..., 3:6], 0.01)   (linear radiance;
+       the loss takes x as is, the denoised image is max(x, 0))
 pool = 2x2 mean, up = nearest. Parameters, flat, per layer in this order:
 W[cout][ky][kx][cin] then b[cout].
 
-Features f (10 channels) from a render: log1p(max(radiance, 0)) x3, albedo x3,
-normal x3, log1p(max(depth, 0)) / 4.
+Features f (12 channels) from a render: log1p(max(radiance, 0) / max(albedo, 0.01))
+x3 (albedo-demodulated), albedo x3, normal x3, log1p(max(depth, 0)) / 4,
+log1p(standard error of the mean luma), log2(spp) / 10. Raw inputs per pixel
+(12 floats): radiance rgb, albedo rgb, normal xyz, depth, mean of luma^2 over
+the samples, sample count.
 
 Loss = mean over pixels and RGB of (x - r)^2 / (r^2 + 0.01)  (the relMSE that
 the quality metric uses).
@@ -42,7 +46,7 @@ import numpy as np
 
 # name, [(source, channels), ...], cout, kernel size, level
 LAYERS = [
-    ("e1", [("f", 10)], 32, 3, 0),
+    ("e1", [("f", 12)], 32, 3, 0),
     ("e2", [("pool:e1", 32)], 64, 3, 1),
     ("e3", [("pool:e2", 64)], 64, 3, 2),
     ("e4", [("pool:e3", 64)], 96, 3, 3),
@@ -85,11 +89,19 @@ def bf16(a: np.ndarray) -> np.ndarray:
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def features(radiance, albedo, normal, depth) -> np.ndarray:
-    """(..., 10) features from render buffers (radiance/albedo/normal (..., 3), depth (...))."""
-    f = np.concatenate([np.log1p(np.maximum(radiance, 0.0)), albedo, normal,
-                        (np.log1p(np.maximum(depth, 0.0)) / 4.0)[..., None]], axis=-1)
-    return f.astype(np.float64)
+def features(raw, bf=False) -> np.ndarray:
+    """(..., 12) features of (..., 12) raw inputs. With bf=True the albedo is
+    rounded to bfloat16 before it demodulates the radiance (as the kernels do)."""
+    raw = np.asarray(raw, np.float64)
+    rad, alb, nrm, dep, lsq, n = raw[..., 0:3], raw[..., 3:6], raw[..., 6:9], raw[..., 9], raw[..., 10], raw[..., 11]
+    m = np.maximum(bf16(alb) if bf else alb, 0.01)
+    n = np.maximum(n, 1.0)
+    luma = rad[..., 0] * 0.2126 + rad[..., 1] * 0.7152 + rad[..., 2] * 0.0722
+    se = np.sqrt(np.maximum(lsq - luma * luma, 0.0) / n)
+    f = np.concatenate([np.log1p(np.maximum(rad, 0.0) / m), alb, nrm,
+                        (np.log1p(np.maximum(dep, 0.0)) / 4.0)[..., None], np.log1p(se)[..., None],
+                        (np.log2(n) * 0.1)[..., None]], axis=-1)
+    return f
 
 
 def conv(x, w, b):
@@ -164,8 +176,9 @@ def forward(theta, f, bf=False):
         else:
             act[name] = rnd(np.maximum(z, 0.0))
     y = act["out"] + f[..., 0:3]
-    x = np.expm1(np.clip(y, 0.0, 20.0))
-    return x, {"W": W, "act": act, "inputs": inputs, "y": y}
+    m = np.maximum(f[..., 3:6], 0.01)
+    x = m * np.expm1(np.minimum(y, 20.0))
+    return x, {"W": W, "act": act, "inputs": inputs, "y": y, "m": m}
 
 
 def loss_and_grad(theta, f, target, bf=False):
@@ -176,7 +189,7 @@ def loss_and_grad(theta, f, target, bf=False):
     den = r * r + 0.01
     loss = float(np.sum((x - r) ** 2 / den) / cnt)
     dx = 2.0 * (x - r) / den / cnt
-    dy = dx * np.where((c["y"] > 0.0) & (c["y"] < 20.0), x + 1.0, 0.0)
+    dy = dx * np.where(c["y"] < 20.0, x + c["m"], 0.0)
     W, act, inputs = c["W"], c["act"], c["inputs"]
     grads = {}
     dact = {name: None for name, *_ in LAYERS}

@@ -78,7 +78,7 @@ int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_tempor
     if (e != hipSuccess) return hip_fail(ctx, e, "post p00 launch");
     bool exceeded = p.atrousMaxSpp > 0 && io.total > p.atrousMaxSpp;
     if (p.atrousEnabled && withGbuf && !exceeded && ctx->denoiser == RC_DENOISE_NET && net_ready(ctx)) {
-        int rc = run_net_denoise(ctx, a, io.gbuf, io.galbedo, b, w, h);
+        int rc = run_net_denoise(ctx, a, io.gbuf, io.galbedo, io.variance, io.total, b, w, h);
         if (rc) return rc;
         float4 *tmp = a; a = b; b = tmp;
     } else if (p.atrousEnabled && withGbuf && !exceeded) {

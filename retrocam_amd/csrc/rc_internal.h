@@ -146,8 +146,8 @@ void free_wavefront(Wavefront &w);
 void free_net(Net *n);
 bool net_ready(const rc_ctx *ctx);
 // Denoises the w x h RGBA32F post image src into dst with the frame's G-buffers.
-int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo, float4 *dst, int w,
-                    int h);
+int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo,
+                    const float *variance, int total, float4 *dst, int w, int h);
 
 // Post chain on explicit buffers (rc_capi.hip).
 int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf);

@@ -5,7 +5,7 @@
 // (pathtrace.comp:6-7, 644-647): radiance, first-hit albedo, shading normal and
 // hit distance.
 //
-//   e1 = relu(conv(f))            10 -> 32    level 0 (full resolution)
+//   e1 = relu(conv(f))            12 -> 32    level 0 (full resolution)
 //   e2 = relu(conv(pool(e1)))     32 -> 64    level 1
 //   e3 = relu(conv(pool(e2)))     64 -> 64    level 2
 //   e4 = relu(conv(pool(e3)))     64 -> 96    level 3
@@ -14,9 +14,13 @@
 //   d3 = relu(conv([up(d4), e3])) 128 -> 64   level 2
 //   d2 = relu(conv([up(d3), e2])) 128 -> 32   level 1
 //   d1 = relu(conv([up(d2), e1]))  64 -> 32   level 0
-//   x  = expm1(clamp(conv1x1(d1) + f[0:3], 0, 20))
+//   x  = m * expm1(min(conv1x1(d1) + f[0:3], 20)),  m = max(albedo, 0.01)
+//        (the loss sees x as is; the denoised image is max(x, 0))
 // pool = 2x2 mean, up = nearest, zero 'same' padding. Features f: log1p of the
-// radiance, albedo, normal, log1p(depth)/4 (channels 10..31 are zero padding).
+// albedo-demodulated radiance (radiance / m), albedo, normal, log1p(depth)/4,
+// log1p of the standard error of the mean luma, log2(spp)/10 (channels 12..31
+// are zero padding). Raw inputs per pixel (12 floats): radiance rgb, albedo
+// rgb, normal xyz, depth, mean of luma^2 over the samples, sample count.
 // Loss = mean (x - r)^2 / (r^2 + 0.01) against a high-spp render r.
 //
 // Precision: activations, features and weights are bfloat16 in HBM (NHWC,
@@ -51,7 +55,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int kNumLayers = 10;
-constexpr int kFeatC = 32;  // padded feature channels (10 used)
+constexpr int kFeatC = 32;  // padded feature channels (12 used)
+constexpr int kRawC = 12;   // raw input floats per pixel
 
 // name, cout, cin (real), cin (padded), kernel, level, source a (layer or -1
 // for the features), a upsampled?, a pooled?, source b (skip layer or -1)
@@ -62,7 +67,7 @@ struct LayerSpec {
 };
 // layer ids: 0 e1, 1 e2, 2 e3, 3 e4, 4 e5, 5 d4, 6 d3, 7 d2, 8 d1, 9 out
 constexpr LayerSpec kLayers[kNumLayers] = {
-    {32, 10, 32, 3, 0, -1, 0, 0, 32, -1, 0},
+    {32, 12, 32, 3, 0, -1, 0, 0, 32, -1, 0},
     {64, 32, 32, 3, 1, 0, 0, 1, 32, -1, 0},
     {64, 64, 64, 3, 2, 1, 0, 1, 64, -1, 0},
     {96, 64, 64, 3, 3, 2, 0, 1, 64, -1, 0},
@@ -368,7 +373,9 @@ __global__ void enc_grad(const float *dxpool, const float *dxcat, int ccat, int 
     dz[i] = f2bf(bf2f(y[i]) > 0.0f ? g : 0.0f);
 }
 
-// Head: o = W d1 + b, y = o + f[0:3], x = expm1(clamp(y, 0, 20)).
+// Head: o = W d1 + b, y = o + f[0:3], x = m * expm1(min(y, 20)) with
+// m = max(albedo feature, 0.01). The loss takes x unclamped (a lower clamp
+// would zero the gradient of every pixel it catches); the image is max(x, 0).
 // Inference writes x (cropped to the image) into dst; training adds the loss
 // and the head's parameter gradient into per-block partials and writes
 // dZ(d1) = (W^T dy) * (d1 > 0).
@@ -403,21 +410,23 @@ __global__ void __launch_bounds__(256) head_kernel(HeadArgs A) {
             float o = A.bias[j];
             for (int c = 0; c < 32; ++c) o += bf2f(A.wb[j * 32 + c]) * d1v[c];
             float y = o + bf2f(A.feat[(size_t)p * 32 + j]);
-            float yc = fminf(fmaxf(y, 0.0f), 20.0f);
-            xo[j] = expm1f(yc);
+            float yc = fminf(y, 20.0f);
+            float m = fmaxf(bf2f(A.feat[(size_t)p * 32 + 3 + j]), 0.01f);
+            xo[j] = m * expm1f(yc);
             if (A.target) {
                 float r = A.target[(size_t)p * 3 + j];
                 float den = r * r + 0.01f;
                 float e = xo[j] - r;
                 lossv += e * e / den;
                 float dx = 2.0f * e / den * A.invCount;
-                dy[j] = (y > 0.0f && y < 20.0f) ? dx * (xo[j] + 1.0f) : 0.0f;
+                dy[j] = y < 20.0f ? dx * (xo[j] + m) : 0.0f;
             }
         }
         if (A.dst) {
             int x = p % A.W, q = p / A.W, yy = q % A.H;
             if (x < A.dstW && yy < A.dstH)
-                A.dst[(size_t)yy * A.dstW + x] = make_float4(xo[0], xo[1], xo[2], 1.0f);
+                A.dst[(size_t)yy * A.dstW + x] =
+                    make_float4(fmaxf(xo[0], 0.0f), fmaxf(xo[1], 0.0f), fmaxf(xo[2], 0.0f), 1.0f);
         }
         if (A.target) {
 #pragma unroll
@@ -460,23 +469,31 @@ __global__ void head_reduce(const float *part, int blocks, float invCount, float
 }
 
 // Features from raw render channels (radiance rgb, albedo rgb, normal xyz,
-// depth): log1p(max(rad, 0)), albedo, normal, log1p(max(depth, 0))/4, zero pad.
+// depth, mean luma^2, spp): log1p(max(rad, 0) / max(albedo, 0.01)), albedo,
+// normal, log1p(max(depth, 0))/4, log1p(standard error of the mean luma),
+// log2(spp)/10, zero pad. The albedo is rounded to bfloat16 before it divides,
+// so the head's remodulation uses the same value.
 __device__ __forceinline__ void write_features(uint16_t *f, const float *raw) {
     uint16_t v[kFeatC];
 #pragma unroll
     for (int c = 0; c < kFeatC; ++c) v[c] = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = f2bf(log1pf(fmaxf(raw[c], 0.0f)));
-#pragma unroll
     for (int c = 3; c < 9; ++c) v[c] = f2bf(raw[c]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = f2bf(log1pf(fmaxf(raw[c], 0.0f) / fmaxf(bf2f(v[3 + c]), 0.01f)));
     v[9] = f2bf(log1pf(fmaxf(raw[9], 0.0f)) * 0.25f);
+    const float n = fmaxf(raw[11], 1.0f);
+    const float luma = (raw[0] * 0.2126f + raw[1] * 0.7152f) + raw[2] * 0.0722f;
+    const float var = fmaxf(raw[10] - luma * luma, 0.0f) / n;
+    v[10] = f2bf(log1pf(sqrtf(var)));
+    v[11] = f2bf(log2f(n) * 0.1f);
 #pragma unroll
     for (int c = 0; c < kFeatC; c += 8) *(u32x4 *)(f + c) = *(const u32x4 *)&v[c];
 }
 
 // Training batch: crop b = hash(seed, b) % nCrops, optional x/y flips.
 struct BatchArgs {
-    const float *raw;     // [nCrops][h][w][10]
+    const float *raw;     // [nCrops][h][w][12]
     const float *tgt;     // [nCrops][h][w][3]
     int nCrops, h, w, batch;
     uint32_t seed;
@@ -500,28 +517,29 @@ __global__ void make_batch(BatchArgs A) {
     const int xs = (A.augment && (hsh & 0x10000u)) ? A.w - 1 - x : x;
     const int ys = (A.augment && (hsh & 0x20000u)) ? A.h - 1 - y : y;
     const size_t src = ((size_t)crop * A.h + ys) * A.w + xs;
-    write_features(A.feat + i * kFeatC, A.raw + src * 10);
+    write_features(A.feat + i * kFeatC, A.raw + src * kRawC);
 #pragma unroll
     for (int c = 0; c < 3; ++c) A.target[i * 3 + c] = A.tgt[src * 3 + c];
 }
-// Features of explicit inputs (no cropping): raw [n][h][w][10].
+// Features of explicit inputs (no cropping): raw [n][h][w][12].
 __global__ void raw_features(const float *raw, size_t npix, uint16_t *feat) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < npix) write_features(feat + i * kFeatC, raw + i * 10);
+    if (i < npix) write_features(feat + i * kFeatC, raw + i * kRawC);
 }
 // Features of a rendered frame, padded to (Hp, Wp) by edge replication:
 // radiance from the post image (p00 output), albedo/normal/depth from the
-// RGBA16F G-buffers.
-__global__ void frame_features(const float4 *hdr, const ushort4 *gbuf, const ushort4 *galb, int w, int h, int Wp,
-                               int Hp, uint16_t *feat) {
+// RGBA16F G-buffers, mean luma^2 = variance sum / spp.
+__global__ void frame_features(const float4 *hdr, const ushort4 *gbuf, const ushort4 *galb, const float *variance,
+                               int total, int w, int h, int Wp, int Hp, uint16_t *feat) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Wp * Hp) return;
     const int x = min(i % Wp, w - 1), y = min(i / Wp, h - 1);
     const size_t s = (size_t)y * w + x;
     float4 c = hdr[s];
     ushort4 g = gbuf[s], a = galb[s];
-    float raw[10] = {c.x, c.y, c.z, rc_f16_to_f32(a.x), rc_f16_to_f32(a.y), rc_f16_to_f32(a.z),
-                     rc_f16_to_f32(g.x), rc_f16_to_f32(g.y), rc_f16_to_f32(g.z), rc_f16_to_f32(g.w)};
+    float raw[kRawC] = {c.x, c.y, c.z, rc_f16_to_f32(a.x), rc_f16_to_f32(a.y), rc_f16_to_f32(a.z),
+                        rc_f16_to_f32(g.x), rc_f16_to_f32(g.y), rc_f16_to_f32(g.z), rc_f16_to_f32(g.w),
+                        variance[s] / (float)total, (float)total};
     write_features(feat + (size_t)i * kFeatC, raw);
 }
 
@@ -575,7 +593,7 @@ struct Net {
     int step = 0;
     bool ready = false;
     // dataset
-    float *raw = nullptr, *tgt = nullptr;
+    float *raw = nullptr, *tgt = nullptr;  // [nCrops][h][w][12], [nCrops][h][w][3]
     int nCrops = 0, cropH = 0, cropW = 0;
     // workspace: capacity in level-0 pixels (N*H*W); level k holds cap >> 2k
     size_t wsCap = 0;
@@ -951,14 +969,14 @@ bool net_ready(const rc_ctx *ctx) { return ctx->net && ctx->net->ready; }
 
 // Denoises the post image src (w x h RGBA32F, p00 output) into dst using the
 // frame's G-buffers.
-int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo, float4 *dst, int w,
-                    int h) {
+int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo,
+                    const float *variance, int total, float4 *dst, int w, int h) {
     Net &n = *ctx->net;
     const int Wp = (w + 15) / 16 * 16, Hp = (h + 15) / 16 * 16;
     hipError_t e = ensure_ws(ctx, n, 1, Hp, Wp, false);
     if (e != hipSuccess) return hip_fail(ctx, e, "net workspace");
     hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)Wp * Hp)), dim3(256), 0, ctx->stream, src, gbuf,
-                       galbedo, w, h, Wp, Hp, n.feat);
+                       galbedo, variance, total, w, h, Wp, Hp, n.feat);
     if ((e = forward(ctx, n, 1, Hp, Wp)) != hipSuccess) return hip_fail(ctx, e, "net forward");
     HeadArgs hd{};
     hd.d1 = n.act[8];
@@ -1030,9 +1048,9 @@ int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32
     nfree(net->raw);
     nfree(net->tgt);
     const size_t px = (size_t)n * h * w;
-    if (e == hipSuccess) e = nalloc(net->raw, px * 10);
+    if (e == hipSuccess) e = nalloc(net->raw, px * kRawC);
     if (e == hipSuccess) e = nalloc(net->tgt, px * 3);
-    if (e == hipSuccess) e = hipMemcpyAsync(net->raw, raw, px * 40, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(net->raw, raw, px * kRawC * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(net->tgt, target, px * 12, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -1111,8 +1129,8 @@ int rc_net_grad(rc_ctx *ctx, const float *raw, const float *target, int32_t n, i
     }
     const size_t px = (size_t)n * h * w;
     float *rawDev = nullptr;
-    if (e == hipSuccess) e = nalloc(rawDev, px * 10);
-    if (e == hipSuccess) e = hipMemcpyAsync(rawDev, raw, px * 40, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = nalloc(rawDev, px * kRawC);
+    if (e == hipSuccess) e = hipMemcpyAsync(rawDev, raw, px * kRawC * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(net.target, target, px * 12, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(raw_features, dim3(blocks_for(px)), dim3(256), 0, ctx->stream, rawDev, px, net.feat);
@@ -1138,9 +1156,9 @@ int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t 
     const size_t px = (size_t)n * h * w;
     float *rawDev = nullptr;
     float4 *outDev = nullptr;
-    if (e == hipSuccess) e = nalloc(rawDev, px * 10);
+    if (e == hipSuccess) e = nalloc(rawDev, px * kRawC);
     if (e == hipSuccess) e = nalloc(outDev, px);
-    if (e == hipSuccess) e = hipMemcpyAsync(rawDev, raw, px * 40, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rawDev, raw, px * kRawC * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(raw_features, dim3(blocks_for(px)), dim3(256), 0, ctx->stream, rawDev, px, net.feat);
         e = forward(ctx, net, n, h, w);

@@ -3,8 +3,9 @@
 Project-new: the reference denoises only with the a-trous filter
 (src/main/java/com/retrocam/post/AtrousDenoiser.java:60-125), which stays the
 fallback. The network's per-pixel inputs are the reference's render outputs
-(pathtrace.comp:6-7, 644-647): the accumulated radiance mean, first-hit albedo,
-shading normal and hit distance ("raw", 10 floats per pixel).
+(pathtrace.comp:6-7, 644-654): the accumulated radiance mean, first-hit albedo,
+shading normal, hit distance, mean luma^2 and the sample count ("raw", 12
+floats per pixel).
 
   NetDenoiser     parameters, training, forward, and the switch that makes
                   rc_post run the network instead of a-trous
@@ -22,6 +23,7 @@ import numpy as np
 from . import _native as N
 from .renderer import Renderer, SceneUploader
 
+RAW_CHANNELS = 12
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_denoiser.npy")
 
 
@@ -64,7 +66,7 @@ class NetDenoiser:
         raw = np.ascontiguousarray(raw, np.float32)
         target = np.ascontiguousarray(target, np.float32)
         n, h, w, c = raw.shape
-        assert c == 10 and target.shape == (n, h, w, 3)
+        assert c == RAW_CHANNELS and target.shape == (n, h, w, 3)
         self.ctx.check(self._lib.rc_net_set_dataset(self.ctx.h, _fp(raw), _fp(target), n, h, w))
 
     def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
@@ -99,17 +101,10 @@ def _half(a: np.ndarray) -> np.ndarray:
 
 
 def frame_raw(r: Renderer, width: int, height: int) -> np.ndarray:
-    """(height, width, 10) raw inputs of the frame in r: radiance mean, albedo,
-    normal, depth (bottom-up rows, as the reference stores images)."""
-    acc = r.read(N.RC_BUF_ACCUM)
-    gb = _half(r.read(N.RC_BUF_GBUFFER))
-    ga = _half(r.read(N.RC_BUF_GALBEDO))
-    raw = np.zeros((height * width, 10), np.float32)
-    raw[:, 0:3] = acc[:, 0:3] / acc[:, 3:4]
-    raw[:, 3:6] = ga[:, 0:3]
-    raw[:, 6:9] = gb[:, 0:3]
-    raw[:, 9] = gb[:, 3]
-    return raw.reshape(height, width, 10)
+    """(height, width, 12) raw inputs of the frame in r (bottom-up rows, as the
+    reference stores images)."""
+    return _raw_from(r.read(N.RC_BUF_ACCUM), r.read(N.RC_BUF_GBUFFER), r.read(N.RC_BUF_GALBEDO),
+                     r.read(N.RC_BUF_VARIANCE), width, height)
 
 
 def render_pair(ctx, w, spp_in: int, spp_ref: int):
@@ -133,18 +128,21 @@ def render_pair(ctx, w, spp_in: int, spp_ref: int):
     return raw, tgt
 
 
-def _raw_from(acc, gbuf, galb, w, h):
+def _raw_from(acc, gbuf, galb, variance, w, h):
+    """Raw inputs: radiance mean, albedo, normal, depth, mean luma^2, spp."""
     gb = _half(gbuf)
     ga = _half(galb)
-    raw = np.zeros((h * w, 10), np.float32)
+    raw = np.zeros((h * w, RAW_CHANNELS), np.float32)
     raw[:, 0:3] = acc[:, 0:3] / acc[:, 3:4]
     raw[:, 3:6] = ga[:, 0:3]
     raw[:, 6:9] = gb[:, 0:3]
     raw[:, 9] = gb[:, 3]
-    return raw.reshape(h, w, 10)
+    raw[:, 10] = variance / acc[:, 3]
+    raw[:, 11] = acc[:, 3]
+    return raw.reshape(h, w, RAW_CHANNELS)
 
 
-def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32), spp_ref: int = 4096, log=None,
+def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256), spp_ref: int = 1024, log=None,
                  group: int = N.RC_MAX_SCENE_SLOTS):
     """Pairs over workloads.training_scene(seed); spp_in cycles through the given
     counts. Views are rendered in batches of `group` scenes, one persistent
@@ -162,7 +160,7 @@ def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32), spp_ref: in
         vb.render(reqs, reset=True)
         for k in range(len(ws)):
             raws.append(_raw_from(vb.read(k, N.RC_BUF_ACCUM), vb.read(k, N.RC_BUF_GBUFFER),
-                                  vb.read(k, N.RC_BUF_GALBEDO), size, size))
+                                  vb.read(k, N.RC_BUF_GALBEDO), vb.read(k, N.RC_BUF_VARIANCE), size, size))
         done = 0
         while done < spp_ref:
             kk = min(512, spp_ref - done)

@@ -1,11 +1,12 @@
 """Trains the learned denoiser on the GPU and writes its weights.
 
-    python -m retrocam_amd.train_denoiser [--scenes 200] [--size 128] [--ref-spp 4096]
-        [--steps 4000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
+    python -m retrocam_amd.train_denoiser [--scenes 384] [--size 128] [--ref-spp 1024]
+        [--steps 12000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
 
 Training data: seeded views from workloads.training_scene rendered by this
-package's path tracer (inputs at 4/8/16/32 spp cycling over the scenes, targets
-at --ref-spp with FP64 accumulation). The optimizer runs entirely in
+package's path tracer (inputs at 4..256 spp cycling over the scenes, targets
+at --ref-spp with FP64 accumulation; the loss is minimised by the targets'
+mean, so their remaining noise averages out over the dataset). The optimizer runs entirely in
 libretrocam_hip.so (rc_net_train: forward, backward and Adam kernels).
 Evaluation: relMSE on held-out views (seeds >= 1,000,000) against their
 high-spp renders, for the noisy input, a-trous and the network.
@@ -21,43 +22,43 @@ import numpy as np
 
 from . import _native as N
 from . import workloads
-from .denoiser import WEIGHTS_PATH, NetDenoiser, make_dataset, rel_mse, render_pair
-from .post import PostProcessStack, TemporalInputs, make_params
+from .batch import ViewBatch, ViewRequest
+from .denoiser import WEIGHTS_PATH, NetDenoiser, make_dataset, rel_mse
+from .post import TemporalInputs, make_params
 from .renderer import RenderContext
 
 
-def atrous_of(ctx):
-    """a-trous (5 passes, default settings) of the frame in ctx's accumulation."""
-    post = PostProcessStack(ctx)
-    return post.run_on_accum(make_params(atrous=True, iterations=5), TemporalInputs()).hdr[:, :3]
-
-
 def evaluate(ctx, net, seeds, size, spp_in, ref_spp):
+    """relMSE of the noisy input, a-trous (5 passes, default settings) and the
+    network on views the training never saw, against ref_spp renders."""
+    seeds = list(seeds)
+    raw, tgt = make_dataset(ctx, seeds, size=size, spp_in=(spp_in,), spp_ref=ref_spp)
+    vb = ViewBatch(ctx)
+    ws = [workloads.training_scene(int(s), size=size) for s in seeds]
+    for k, w in enumerate(ws):
+        vb.upload_scene(k, w.scene)
+    vb.render([ViewRequest(k, w.camera, w.params, spp_in) for k, w in enumerate(ws)], reset=True)
+    pp = make_params(atrous=True, iterations=5)
     rows = []
-    for s in seeds:
-        w = workloads.training_scene(int(s), size=size)
-        raw, tgt = render_pair(ctx, w, spp_in, ref_spp)
-        # à-trous on the same low-spp frame (render_pair leaves the reference in the buffers)
-        from .renderer import Renderer
-        r = Renderer(ctx)
-        r.reset()
-        r.render(w.camera, w.params, n_samples=spp_in)
-        at = atrous_of(ctx).reshape(size, size, 3)
-        out = net.forward(raw[None])[0]
-        rows.append((rel_mse(raw[..., 0:3], tgt), rel_mse(at, tgt), rel_mse(out, tgt)))
+    for k in range(len(ws)):
+        vb.post(k, pp, TemporalInputs())
+        at = vb.read(k, N.RC_BUF_POST_HDR)[:, :3].reshape(size, size, 3)
+        out = net.forward(raw[k:k + 1])[0]
+        rows.append((rel_mse(raw[k, ..., 0:3], tgt[k]), rel_mse(at, tgt[k]), rel_mse(out, tgt[k])))
     return np.array(rows)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=200)
+    ap.add_argument("--scenes", type=int, default=384)
     ap.add_argument("--size", type=int, default=128)
-    ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--ref-spp", type=int, default=1024)
+    ap.add_argument("--eval-ref-spp", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--eval-scenes", type=int, default=8)
+    ap.add_argument("--eval-scenes", type=int, default=16)
     ap.add_argument("--out", default=WEIGHTS_PATH)
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
     args = ap.parse_args(argv)
@@ -88,7 +89,7 @@ def main(argv=None):
     losses = np.concatenate(losses)
     net.save(args.out)
     log(f"[train] {args.steps} steps in {t_train:.1f} s; weights -> {args.out}")
-    ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.ref_spp)
+    ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.eval_ref_spp)
     summary = {
         "scenes": args.scenes, "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
         "batch": args.batch, "lr": args.lr, "params": net.n,

@@ -21,19 +21,21 @@ pytestmark = pytest.mark.gpu
 
 def random_raw(seed, n, h, w):
     rng = np.random.default_rng(seed)
-    raw = np.zeros((n, h, w, 10), np.float32)
+    raw = np.zeros((n, h, w, 12), np.float32)
     raw[..., 0:3] = rng.gamma(1.0, 0.5, (n, h, w, 3))
     raw[..., 3:6] = rng.uniform(0.05, 0.9, (n, h, w, 3))
     nrm = rng.standard_normal((n, h, w, 3))
     raw[..., 6:9] = nrm / np.linalg.norm(nrm, axis=-1, keepdims=True)
     raw[..., 9] = rng.uniform(0.5, 12.0, (n, h, w))
+    luma = raw[..., 0] * 0.2126 + raw[..., 1] * 0.7152 + raw[..., 2] * 0.0722
+    raw[..., 10] = luma * luma + rng.uniform(0.0, 2.0, (n, h, w))
+    raw[..., 11] = rng.choice([4.0, 8.0, 64.0], (n, h, w))
     tgt = (raw[..., 0:3] * rng.uniform(0.7, 1.3, (n, h, w, 3))).astype(np.float32)
     return raw, tgt
 
 
 def oracle_features(raw):
-    r = raw.astype(np.float64)
-    return U.features(r[..., 0:3], r[..., 3:6], r[..., 6:9], r[..., 9])
+    return U.features(raw.astype(np.float64), bf=True)
 
 
 def test_forward_matches_oracle(gpu_ctx):
@@ -42,6 +44,7 @@ def test_forward_matches_oracle(gpu_ctx):
     raw, _ = random_raw(1, 2, 32, 48)
     out = net.forward(raw)
     ref, _ = U.forward(theta.astype(np.float64), oracle_features(raw), bf=True)
+    ref = np.maximum(ref, 0.0)  # the denoised image is clamped at 0
     assert out.shape == ref.shape
     assert rel_mse(out, ref) <= 1e-4
 

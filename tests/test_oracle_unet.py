@@ -9,14 +9,16 @@ from retrocam_amd import _native as N
 
 def small_problem(seed=0, n=1, h=16, w=16):
     rng = np.random.default_rng(seed)
-    f = U.features(rng.uniform(0, 3, (n, h, w, 3)), rng.uniform(0, 1, (n, h, w, 3)),
-                   rng.standard_normal((n, h, w, 3)), rng.uniform(0, 10, (n, h, w)))
+    raw = np.concatenate([rng.uniform(0, 3, (n, h, w, 3)), rng.uniform(0, 1, (n, h, w, 3)),
+                          rng.standard_normal((n, h, w, 3)), rng.uniform(0, 10, (n, h, w, 1)),
+                          rng.uniform(0, 9, (n, h, w, 1)), np.full((n, h, w, 1), 8.0)], axis=-1)
+    f = U.features(raw)
     tgt = rng.uniform(0, 3, (n, h, w, 3))
     return f, tgt
 
 
 def test_param_count_matches_library():
-    assert U.param_count() == 482787
+    assert U.param_count() == 482787 + 2 * 9 * 32
     assert N.hip_lib().rc_net_param_count() == U.param_count()
 
 
@@ -77,3 +79,19 @@ def test_bf16_rounding():
     assert r[3] == 1.0 + 2 ** -7            # exact
     assert r[4] == -2.5
     assert np.all(np.abs(r[:5] - v[:5]) <= np.abs(v[:5]) * 2 ** -8)
+
+
+def test_features_closed_forms():
+    raw = np.zeros((1, 12))
+    raw[0, 0:3] = [0.5, 1.0, 2.0]
+    raw[0, 3:6] = [0.5, 0.25, 0.0]
+    raw[0, 6:9] = [0.0, 1.0, 0.0]
+    raw[0, 9] = np.e - 1.0
+    luma = 0.5 * 0.2126 + 1.0 * 0.7152 + 2.0 * 0.0722
+    raw[0, 10] = luma * luma + 4.0 * 16.0   # sample variance of luma 64 over 16 samples -> se 2
+    raw[0, 11] = 16.0
+    f = U.features(raw)[0]
+    assert np.allclose(f[0:3], np.log1p([1.0, 4.0, 200.0]))   # demodulated by max(albedo, 0.01)
+    assert np.allclose(f[9], 0.25)
+    assert np.isclose(f[10], np.log1p(2.0))
+    assert np.isclose(f[11], 0.4)
