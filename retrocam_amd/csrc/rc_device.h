@@ -138,7 +138,9 @@ struct Hit {
 // (row-major [entry][lane] so a wave's accesses hit 64 distinct banks); deeper
 // entries spill to a private array. The logical stack depth and the reference's
 // caps (62 closest-hit, 30 shadow) are unchanged by where entries live.
-constexpr int kShortStack = 12;
+// 11 entries x 8 B x 256 lanes = 22.5 KB per workgroup: seven workgroups (7
+// waves per SIMD) fit the 160 KB LDS of a CU (measured best on the full frame).
+constexpr int kShortStack = 11;
 constexpr int kBlock = 256;
 
 struct StackRef {

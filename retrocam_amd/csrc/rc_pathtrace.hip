@@ -161,8 +161,10 @@ __device__ __forceinline__ void trace_pixel(const PassArgs &A, int px, int py, c
     }
 }
 
+// Occupancy: 7 waves per SIMD (<= 72 VGPRs) beats the unconstrained 4 waves
+// (111 VGPRs) by ~20% on the full frame; the divergent traversal is latency bound.
 template <bool STATS, bool FP64>
-__global__ void __launch_bounds__(kBlock) pathtrace_kernel(PassArgs A) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) pathtrace_kernel(PassArgs A) {
     __shared__ int2 stk[kShortStack][kBlock];
     const StackRef st{stk, (int)threadIdx.x};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) pathtrace_kernel(PassArgs A) {
 // tiles of all views are done. tileOff[v] is the first tile of view v (views
 // are laid out heaviest first, so long tiles start early).
 template <bool STATS, bool FP64>
-__global__ void __launch_bounds__(kBlock) pathtrace_views_kernel(const PassArgs *views, const int *tileOff,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) pathtrace_views_kernel(const PassArgs *views, const int *tileOff,
                                                                  int nViews, int *counter,
                                                                  unsigned long long *counters) {
     __shared__ int2 stk[kShortStack][kBlock];
