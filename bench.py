@@ -110,21 +110,94 @@ def cpu_baseline(w, spp_sample: int, spp_frame: int):
             "mrays_per_s": rays / dt / 1e6}
 
 
-def read_traffic():
-    """HBM bytes per path-trace launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size)."""
-    path = os.path.join(ROOT, "profiles", "pathtrace_kernel__full_frame__pmc.md")
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(float(line.split(":", 1)[1]))
-    except OSError:
-        pass
-    return None
+class EventClock:
+    """hipEvents on one stream; elapsed milliseconds of each mark since the first."""
+
+    def __init__(self, stream):
+        self.t = HipTimer(stream)
+        self.events = []
+
+    def mark(self):
+        e = self.t._ev()
+        self.t.hip.hipEventRecord(e, self.t.stream)
+        self.events.append(e)
+
+    def elapsed_ms(self):
+        hip = self.t.hip
+        hip.hipEventSynchronize(self.events[-1])
+        out = []
+        for e in self.events[1:]:
+            ms = C.c_float()
+            hip.hipEventElapsedTime(C.byref(ms), self.events[0], e)
+            out.append(ms.value)
+        for e in self.events:
+            hip.hipEventDestroy(e)
+        self.events = []
+        return out
 
 
-def rel_mse(x, r):
-    return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
+def serving(ctx, net, batches, log):
+    """32 uneven view requests (4 scenes, 4 resolutions, spp U{4..128}) served as
+    one batch — a single persistent path-trace launch, then per view the U-Net,
+    VHS chain and display — against the same requests served one at a time.
+    Every request arrives at t = 0; latency = time until its display image is
+    ready (hipEvents on the library stream)."""
+    from retrocam_amd import workloads
+    from retrocam_amd.batch import ViewBatch, ViewRequest
+    from retrocam_amd.post import VHS_PASSES, TemporalInputs, make_params
+    scenes, reqs = workloads.serving_batch()
+    vb = ViewBatch(ctx)
+    for k, sc in enumerate(scenes):
+        vb.upload_scene(k, sc)
+    vreqs = [ViewRequest(slot, cam, params, spp) for slot, cam, params, spp in reqs]
+    pp = make_params(enable=VHS_PASSES, atrous=True, iterations=5)
+    tmp = TemporalInputs()
+    net.enable(True)
+
+    def batch_once():
+        clk = EventClock(ctx.stream)
+        clk.mark()
+        vb.render(vreqs, reset=True)
+        for i in range(len(vreqs)):
+            vb.post(i, pp, tmp)
+            clk.mark()
+        return clk.elapsed_ms()
+
+    def sequential_once():
+        clk = EventClock(ctx.stream)
+        clk.mark()
+        for q in vreqs:
+            vb.render([q], reset=True)
+            vb.post(0, pp, tmp)
+            clk.mark()
+        return clk.elapsed_ms()
+
+    batch_once()  # warm-up: arenas and the network workspace reach their largest size
+    lat_b, lat_s, tb, ts = [], [], [], []
+    for b in range(batches):
+        lat = batch_once()
+        lat_b += lat
+        tb.append(lat[-1])
+        log(f"[bench] serving batch {b}: {lat[-1]:.1f} ms for {len(vreqs)} requests")
+    for b in range(batches):
+        lat = sequential_once()
+        lat_s += lat
+        ts.append(lat[-1])
+        log(f"[bench] sequential pass {b}: {lat[-1]:.1f} ms for {len(vreqs)} requests")
+    net.enable(False)
+    px = sum(q.camera.width * q.camera.height * q.spp for q in vreqs)
+    return {
+        "requests": len(vreqs), "batches": batches, "scene_triangles": [int(s.nTris) for s in scenes],
+        "pixel_samples_per_batch": int(px),
+        "batched": {"requests_per_s": round(len(vreqs) / (float(np.mean(tb)) / 1e3), 3),
+                    "p50_ms": round(float(np.percentile(lat_b, 50)), 1),
+                    "p99_ms": round(float(np.percentile(lat_b, 99)), 1), "batch_ms": round(float(np.mean(tb)), 1)},
+        "sequential": {"requests_per_s": round(len(vreqs) / (float(np.mean(ts)) / 1e3), 3),
+                       "p50_ms": round(float(np.percentile(lat_s, 50)), 1),
+                       "p99_ms": round(float(np.percentile(lat_s, 99)), 1), "pass_ms": round(float(np.mean(ts)), 1)},
+        "speedup": round(float(np.mean(ts)) / float(np.mean(tb)), 3),
+        "chain": "one persistent trace launch for all views, then per view: p00, U-Net, 14 VHS passes, display",
+    }
 
 
 def read_traffic():
@@ -156,6 +229,7 @@ def main():
                     help="spp values searched for the relMSE <= 0.01 operating points")
     ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["wavefront", "megakernel"], default="megakernel")
+    ap.add_argument("--serving", type=int, default=1, help="batched-serving batches to time (0 = skip)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 1 GiB device copies (calibrates FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc)")
     ap.add_argument("--quiet", action="store_true")
@@ -307,6 +381,11 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copy calibration: 3 x 1 GiB device-to-device copies done")
 
+    serve = None
+    if args.serving > 0:
+        serve = serving(ctx, net, args.serving, log)
+        log(f"[bench] serving: {serve}")
+
     cpu = None
     if args.cpu_spp > 0:
         log(f"[bench] CPU baseline: oracle, {args.cpu_spp} spp ...")
@@ -340,6 +419,7 @@ def main():
         "post_ms": round(post_ms, 4),
         "plain_128spp_atrous_frames_per_s": plain128,
         "quality": quality,
+        "serving": serve,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(traffic128 * spp / 128) if traffic128 else None,

@@ -7,6 +7,7 @@
                   (50% diffuse, 30% GGX metal alpha U[0.05, 0.6], 20% glass
                   ior 1.5) in a closed room with 8 quad area lights; thin lens
                   35 mm f/2.8, hexagon bokeh, focus 2.0 m, SA 0.04.
+  serving_batch   4 scenes (50k-250k triangles) + 32 uneven view requests.
   training_scene  seeded size x size denoiser-training views (rooms with random
                   objects, lights and lenses).
   reference_default  the reference's default Cornell scene
@@ -194,6 +195,29 @@ def training_scene(seed: int, size: int = 128, spp: int = 8) -> Workload:
     else:
         cam = lookat_camera(size, size, eye=eye, target=tgt, pinhole=True)
     return Workload("training_scene", scene, cam, PtParams(1, 1.0, 6), spp, {"seed": seed})
+
+
+def serving_batch(seed: int = 1, n_requests: int = 32):
+    """Batched-serving workload: 4 seeded full-frame-style scenes of ~50k, 100k,
+    150k and 250k triangles and n_requests view requests with resolutions drawn
+    from {320x180, 640x360, 854x480, 1280x720}, spp from U{4..128}, a random
+    scene and a jittered thin-lens camera. Returns (scenes, [(slot, camera,
+    params, spp)])."""
+    sizes = (50_008, 100_012, 150_004, 250_000)  # room + lights = 784 triangles, objects in 12s
+    scenes = [full_frame(seed=21 + i, n_tris=n, spp=1).scene for i, n in enumerate(sizes)]
+    rng = np.random.default_rng(1000 + seed)
+    res = [(320, 180), (640, 360), (854, 480), (1280, 720)]
+    reqs = []
+    for _ in range(n_requests):
+        w, h = res[int(rng.integers(0, 4))]
+        spp = int(rng.integers(4, 129))
+        slot = int(rng.integers(0, 4))
+        eye = (float(rng.uniform(-3, 3)), float(rng.uniform(1.2, 3.0)), float(rng.uniform(5.0, 7.5)))
+        tgt = (float(rng.uniform(-2, 2)), float(rng.uniform(0.5, 1.5)), float(rng.uniform(-2, 1)))
+        cam = lookat_camera(w, h, eye=eye, target=tgt, focalDist=float(rng.uniform(1.5, 5.0)), focalLengthMm=35.0,
+                            fStop=2.8, blades=6, saStrength=0.04)
+        reqs.append((slot, cam, PtParams(1, 1.0, 6), spp))
+    return scenes, reqs
 
 
 def reference_default(width: int = 854, height: int = 480, spp: int = 16) -> Workload:
