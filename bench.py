@@ -200,14 +200,63 @@ def serving(ctx, net, batches, log):
     }
 
 
+def sppm_leg(ctx, iters, log):
+    """SPPM caustics frame (RenderPipeline.accumulateFrame with SPPM,
+    src/main/java/com/retrocam/export/RenderPipeline.java:209-239): per
+    iteration photon trace (2,000,000 photons, hash 2^22), one path-trace pass,
+    gather and radius update (r0 0.02, alpha 0.7, FP64 radius schedule), on the
+    Cornell scene plus three glass spheres at 854x480."""
+    from retrocam_amd import workloads
+    from retrocam_amd.renderer import Renderer, SceneUploader
+    from retrocam_amd.sppm import SPPMManager
+    w = workloads.sppm_caustics()
+    SceneUploader(ctx).upload(w.scene)
+    r = Renderer(ctx)
+    s = SPPMManager(ctx, init_radius=w.extra["init_radius"], fp64_radius=True)
+    P, Hs, alpha = w.extra["photons"], w.extra["hash_size"], w.extra["alpha"]
+
+    def run(n, timed):
+        tp, tt, tg = HipTimer(ctx.stream), HipTimer(ctx.stream), HipTimer(ctx.stream)
+        r.reset()
+        s.reset(w.extra["init_radius"])
+        stored = []
+        for _ in range(n):
+            tp.start(); s.trace_photons(P, Hs); tp.stop()
+            if timed and len(stored) < 4:
+                stored.append(s.stored_photons())
+            tt.start(); r.render(w.camera, w.params, n_samples=1); tt.stop()
+            tg.start(); s.gather_radiance(w.camera); tg.stop()
+            s.update_radius(alpha)
+        return tp.times_ms(), tt.times_ms(), tg.times_ms(), stored
+
+    run(2, False)
+    t0 = time.perf_counter()
+    tp, tt, tg, stored = run(iters, True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    npx = w.camera.width * w.camera.height
+    st = int(np.mean(stored))
+    ins_bytes = st * (48 + 4 + 8) + Hs * 4  # record + next + atomicExch RMW per stored photon, heads clear
+    out = {"iterations": iters, "photons_per_iteration": P, "hash_cells": Hs, "stored_photons_per_iteration": st,
+           "frame_s": round(wall, 3), "photon_trace_ms": round(float(np.mean(tp)), 3),
+           "path_trace_ms": round(float(np.mean(tt)), 3), "gather_ms": round(float(np.mean(tg)), 3),
+           "mphotons_per_s": round(P / (float(np.mean(tp)) / 1e3) / 1e6, 1),
+           "gather_mpixels_per_s": round(npx / (float(np.mean(tg)) / 1e3) / 1e6, 1),
+           "insert_gbs_compulsory": round(ins_bytes / (float(np.mean(tp)) / 1e3) / 1e9, 1),
+           "radius_final": s.search_radius}
+    log(f"[bench] sppm: {out}")
+    return out
+
+
 def read_traffic():
-    """HBM bytes per path-trace launch of 128 spp from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size)."""
+    """HBM bytes per sample of a full-frame path-trace launch, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of
+    known size, divided by the launch's samples)."""
     path = os.path.join(ROOT, "profiles", "pathtrace_kernel__full_frame__pmc.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(float(line.split(":", 1)[1]))
+            if line.startswith("traffic_bytes_per_spp:"):
+                return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
@@ -223,13 +272,15 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fixed-spp", type=int, default=0,
-                    help="time frames at this spp with a-trous (skips the quality search when --ref-spp 0)")
+                    help="time frames at this spp (with --denoiser) instead of the U-Net's operating point")
+    ap.add_argument("--denoiser", choices=["atrous", "net"], default="atrous", help="denoiser of --fixed-spp frames")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
     ap.add_argument("--sweep", default="4,8,16,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
                     help="spp values searched for the relMSE <= 0.01 operating points")
     ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["wavefront", "megakernel"], default="megakernel")
     ap.add_argument("--serving", type=int, default=1, help="batched-serving batches to time (0 = skip)")
+    ap.add_argument("--sppm-iters", type=int, default=64, help="SPPM caustics iterations to time (0 = skip)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 1 GiB device copies (calibrates FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc)")
     ap.add_argument("--quiet", action="store_true")
@@ -330,7 +381,7 @@ def main():
     if "net" in op and not args.fixed_spp:
         spp, method = op["net"]["spp"], "net"
     else:
-        spp, method = (args.fixed_spp or 128), "atrous"
+        spp, method = (args.fixed_spp or 128), (args.denoiser if args.fixed_spp else "atrous")
     quality_met = bool(quality and method in op and op[method]["spp"] == spp)
 
     # Counting pass (separately compiled instrumented kernel, identical results):
@@ -369,7 +420,7 @@ def main():
     post_ms = float(np.mean(pms))
     log(f"[bench] per-frame trace ms: {[round(x, 2) for x in kms]}; post chain ms: {[round(x, 3) for x in pms]}")
 
-    plain128 = round(1e3 / time_frames(128, "atrous", 3), 4)
+    plain128 = round(1e3 / time_frames(128, "atrous", 3), 4) if quality else None
     net.enable(False)
 
     if args.copy_calib:
@@ -381,6 +432,8 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copy calibration: 3 x 1 GiB device-to-device copies done")
 
+    sppm = sppm_leg(ctx, args.sppm_iters, log) if args.sppm_iters > 0 else None
+
     serve = None
     if args.serving > 0:
         serve = serving(ctx, net, args.serving, log)
@@ -391,7 +444,7 @@ def main():
         log(f"[bench] CPU baseline: oracle, {args.cpu_spp} spp ...")
         cpu = cpu_baseline(w, args.cpu_spp, spp)
 
-    traffic128 = read_traffic()
+    traffic_spp = read_traffic()
     out = {
         "metric": "854x480 frames/s at relMSE<=0.01 vs 4096spp (net vs a-trous); Mrays/s",
         "value": round(fps, 4),
@@ -420,9 +473,10 @@ def main():
         "plain_128spp_atrous_frames_per_s": plain128,
         "quality": quality,
         "serving": serve,
+        "sppm": sppm,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": int(traffic128 * spp / 128) if traffic128 else None,
+                     "traffic": int(traffic_spp * spp) if traffic_spp else None,
                      "kernel": "pathtrace_kernel" if args.pt_mode == "megakernel" else "wavefront frame",
                      "alg_bytes_per_launch": int(alg_bytes),
                      "avg_launch_ms": round(avg_kernel_s * 1e3, 3)},

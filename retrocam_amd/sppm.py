@@ -53,6 +53,12 @@ class SPPMManager:
     def iteration(self) -> int:
         return self.state()[1]
 
+    def stored_photons(self) -> int:
+        """Photons the last trace stored (the insertion counter, capped by the map size)."""
+        cnt = np.zeros(1, np.uint32)
+        self.ctx.check(self._lib.rc_sppm_read(self.ctx.h, None, None, None, cnt.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return min(int(cnt[0]), self.photon_count)
+
     def read_map(self):
         P, Hs = self.photon_count, self.hash_size
         photons = np.zeros((P, 12), np.float32)

@@ -89,7 +89,8 @@ def main(argv=None):
     losses = np.concatenate(losses)
     net.save(args.out)
     log(f"[train] {args.steps} steps in {t_train:.1f} s; weights -> {args.out}")
-    ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.eval_ref_spp)
+    ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.eval_ref_spp) \
+        if args.eval_scenes > 0 else np.full((1, 3), np.nan)
     summary = {
         "scenes": args.scenes, "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
         "batch": args.batch, "lr": args.lr, "params": net.n,
