@@ -164,10 +164,14 @@ enum {
     RC_ACCUM_FP64 = 1       /* FP64 running sum; RC_BUF_ACCUM returns it rounded   */
 };
 
-/* Path-tracer kernel organisation (rc_set_pt_mode); both give identical results. */
+/* Path-tracer kernel organisation (rc_set_pt_mode); all give identical results. */
 enum {
     RC_PT_MEGAKERNEL = 0,   /* one kernel, one lane per pixel for all samples     */
-    RC_PT_WAVEFRONT = 1     /* extend / shadow / shade kernels over ray queues    */
+    RC_PT_WAVEFRONT = 1,    /* extend / shadow / shade kernels over ray queues    */
+    RC_PT_LANES = 2         /* default: per-lane state machines pulling single
+                               samples from a work counter; used for scenes whose
+                               BVH is shallower than 30 levels, the megakernel
+                               otherwise                                         */
 };
 
 /* Denoiser of the post chain (rc_set_denoiser). */
@@ -219,8 +223,8 @@ int rc_render_passes(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *para
 int rc_reset(rc_ctx *ctx);
 /* Renderer.getTotalSamples (Renderer.java:138). */
 int32_t rc_total_samples(const rc_ctx *ctx);
-/* RC_PT_MEGAKERNEL (default) or RC_PT_WAVEFRONT for later passes; both give
-   bit-identical images. */
+/* RC_PT_LANES (default), RC_PT_MEGAKERNEL or RC_PT_WAVEFRONT for later passes;
+   all give bit-identical images. */
 int rc_set_pt_mode(rc_ctx *ctx, int32_t mode);
 /* RC_ACCUM_FP32 (default) or RC_ACCUM_FP64; resets the accumulation. */
 int rc_set_accum_mode(rc_ctx *ctx, int32_t mode);
@@ -329,6 +333,22 @@ int rc_net_grad(rc_ctx *ctx, const float *raw, const float *target, int32_t n, i
 int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t w, float *out);
 /* Which denoiser rc_post runs when rc_post_params.atrousEnabled is set. */
 int rc_set_denoiser(rc_ctx *ctx, int32_t mode);
+
+/* ---- tuning (no counterpart in the reference) --------------------------- */
+/* Launch parameters that change speed, never results. */
+#define RC_LANE_VARIANTS 6
+typedef struct rc_tuning {
+    int32_t laneThreshold;  /* RC_PT_LANES: a wave stops walking and shades once this
+                               many of its 64 lanes wait (1..64, default 40)       */
+    int32_t laneVariant;    /* RC_PT_LANES kernel variant, 0..RC_LANE_VARIANTS-1:
+                               occupancy, LDS stack depth, triangles per step
+                               (default 4: 7 waves/SIMD, 11 entries, one triangle) */
+    int32_t laneBufferMiB;  /* RC_PT_LANES sample buffer bound (16 B per pixel and
+                               sample; larger launches run in chunks), 64..65536,
+                               default 4096                                        */
+} rc_tuning;
+int rc_get_tuning(const rc_ctx *ctx, rc_tuning *out);
+int rc_set_tuning(rc_ctx *ctx, const rc_tuning *t);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Work counters of the path-trace kernel, for the roofline byte model. Copies

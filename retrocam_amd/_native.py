@@ -38,6 +38,8 @@ RC_ACCUM_FP64 = 1
 
 RC_PT_MEGAKERNEL = 0
 RC_PT_WAVEFRONT = 1
+RC_PT_LANES = 2
+RC_LANE_VARIANTS = 6
 
 RC_DENOISE_ATROUS = 0
 RC_DENOISE_NET = 1
@@ -91,6 +93,10 @@ class RcView(C.Structure):
 RC_VIEWS_RESET = 1
 RC_VIEWS_FP64 = 2
 RC_MAX_SCENE_SLOTS = 64
+
+
+class RcTuning(C.Structure):
+    _fields_ = [("laneThreshold", C.c_int32), ("laneVariant", C.c_int32), ("laneBufferMiB", C.c_int32)]
 
 
 class RcTemporal(C.Structure):
@@ -151,6 +157,8 @@ HIP_SYMBOLS = {
     "rc_net_grad": (C.c_int, [_VP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP, _FP]),
     "rc_net_forward": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP]),
     "rc_set_denoiser": (C.c_int, [_VP, C.c_int32]),
+    "rc_get_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
+    "rc_set_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),
     "rc_debug_pcg": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
     "rc_debug_rays": (C.c_int, [_VP, _FP, C.c_int32, C.POINTER(C.c_int32), _FP, C.POINTER(C.c_int32)]),

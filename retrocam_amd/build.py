@@ -32,7 +32,7 @@ HIP_FLAGS = [
     f"-I{INC}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["rc_capi.hip", "rc_pathtrace.hip", "rc_wavefront.hip", "rc_debug.hip", "rc_post.hip",
+HIP_SOURCES = ["rc_capi.hip", "rc_pathtrace.hip", "rc_pathtrace_lanes.hip", "rc_wavefront.hip", "rc_debug.hip", "rc_post.hip",
                "rc_sppm.hip", "rc_net.hip", "rc_batch.hip"]
 
 

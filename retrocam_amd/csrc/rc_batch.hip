@@ -37,6 +37,10 @@ struct Views {
     int *dCounter = nullptr;
     PassArgs *hArgs = nullptr;  // pinned staging of the table
     int *hTileOff = nullptr;
+    // lane-scheduler tables (rc_pathtrace_lanes.hip): chunk x view entries
+    int chunkCap = 0;
+    LaneChunk *dChunks = nullptr, *hChunks = nullptr;
+    long long *dPixOff = nullptr, *hPixOff = nullptr;
 };
 
 namespace {
@@ -53,7 +57,17 @@ void free_arenas(Views &v) {
     v.pixCap = 0;
 }
 
+void free_lane_tables(Views &v) {
+    vfree(v.dChunks); vfree(v.dPixOff);
+    if (v.hChunks) (void)hipHostFree(v.hChunks);
+    if (v.hPixOff) (void)hipHostFree(v.hPixOff);
+    v.hChunks = nullptr;
+    v.hPixOff = nullptr;
+    v.chunkCap = 0;
+}
+
 void free_tables(Views &v) {
+    free_lane_tables(v);
     vfree(v.dArgs); vfree(v.dTileOff); vfree(v.dCounter);
     if (v.hArgs) (void)hipHostFree(v.hArgs);
     if (v.hTileOff) (void)hipHostFree(v.hTileOff);
@@ -137,6 +151,78 @@ int layout(rc_ctx *ctx, Views &V, const rc_view *views, int n, bool fp64, bool &
     return RC_OK;
 }
 
+// The views of a batch traced by the lane scheduler, in chunks of samples
+// that keep the sample buffer within rc_ctx::laneBufferMiB. hArgs[0..n) is
+// the sorted table already copied to dArgs.
+int render_views_lanes(rc_ctx *ctx, Views &V, int n, bool fp64) {
+    std::vector<long long> tilePx(n);
+    int maxS = 0;
+    for (int k = 0; k < n; ++k) {
+        const CamArgs &c = V.hArgs[k].cam;
+        tilePx[k] = (long long)((c.width + 7) / 8) * ((c.height + 7) / 8) * 64;
+        maxS = std::max(maxS, V.hArgs[k].nSamples);
+    }
+    // largest chunk K whose buffer fits, then whole chunks of K samples per view
+    auto bytesFor = [&](long long K) {
+        long long b = 0;
+        for (int k = 0; k < n; ++k) b += std::min<long long>(K, V.hArgs[k].nSamples) * tilePx[k] * 16;
+        return b;
+    };
+    long long K = maxS;
+    const long long cap = (long long)ctx->laneBufferMiB << 20;
+    while (K > 1 && bytesFor(K) > cap) K = (K + 1) / 2;
+    for (int k = 0; k < n; ++k)
+        while (K > 1 && tilePx[k] * K >= (1ll << 30)) K = (K + 1) / 2;
+    if (tilePx[0] * K >= (1ll << 30)) return set_error(ctx, RC_ERR_LIMIT, "views: view too large");
+    const int nChunks = (int)((maxS + K - 1) / K);
+    const long long entries = (long long)nChunks * n;
+    if (entries > (1 << 24)) return set_error(ctx, RC_ERR_LIMIT, "views: too many chunks");
+    hipError_t e = ensure_lane_buffers(ctx, (size_t)bytesFor(K), (int)entries);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(lane buffers)");
+    if (entries > V.chunkCap || !V.dPixOff || n + 1 > V.viewCap + 1) {
+        free_lane_tables(V);
+        e = valloc(V.dChunks, (size_t)entries);
+        if (e == hipSuccess) e = valloc(V.dPixOff, (size_t)V.viewCap + 1);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&V.hChunks, sizeof(LaneChunk) * entries, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void **)&V.hPixOff, sizeof(long long) * (V.viewCap + 1), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            free_lane_tables(V);
+            return hip_fail(ctx, e, "view lane tables");
+        }
+        V.chunkCap = (int)entries;
+    }
+    long long px = 0, off = 0;
+    for (int k = 0; k < n; ++k) {
+        V.hPixOff[k] = px;
+        px += (long long)V.hArgs[k].cam.width * V.hArgs[k].cam.height;
+        for (int c = 0; c < nChunks; ++c) {
+            LaneChunk &L = V.hChunks[(size_t)c * n + k];
+            const long long s0 = (long long)c * K;
+            const long long ns = std::max(0ll, std::min<long long>(K, V.hArgs[k].nSamples - s0));
+            L.samples = ctx->lanes.samples + off;
+            L.counter = ctx->lanes.counters + (size_t)c * n + k;
+            L.s0 = (int)s0;
+            L.nSamples = (int)ns;
+            L.nTilePx = (int)tilePx[k];
+        }
+        off += std::min<long long>(K, V.hArgs[k].nSamples) * tilePx[k];
+    }
+    V.hPixOff[n] = px;
+    e = hipMemcpyAsync(V.dChunks, V.hChunks, sizeof(LaneChunk) * entries, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(V.dPixOff, V.hPixOff, sizeof(long long) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "view lane tables upload");
+    for (int c = 0; c < nChunks; ++c) {
+        long long units = 0;
+        for (int k = 0; k < n; ++k) units += (long long)V.hChunks[(size_t)c * n + k].nSamples * tilePx[k];
+        e = launch_lanes_views(ctx, V.dArgs, V.dChunks + (size_t)c * n, V.dPixOff, n, px, units, ctx->laneThresh,
+                               ctx->laneVariant, fp64, c == 0 ? (int)entries : 0);
+        if (e != hipSuccess) return hip_fail(ctx, e, "views lane launch");
+    }
+    return RC_OK;
+}
+
 Views *get_views(rc_ctx *ctx) {
     if (!ctx->views) ctx->views = new (std::nothrow) Views();
     return ctx->views;
@@ -209,8 +295,15 @@ int rc_views_render(rc_ctx *ctx, const rc_view *views, int32_t n, int32_t flags)
     e = hipMemcpyAsync(V->dArgs, V->hArgs, sizeof(PassArgs) * n, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(V->dTileOff, V->hTileOff, sizeof(int) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = launch_pathtrace_views(ctx, V->dArgs, V->dTileOff, n, V->dCounter, (int)tot, fp64);
-    if (e != hipSuccess) return hip_fail(ctx, e, "views launch");
+    if (e != hipSuccess) return hip_fail(ctx, e, "views table upload");
+    bool lanes = ctx->ptMode == RC_PT_LANES;
+    for (int i = 0; i < n && lanes; ++i) lanes = lanes_supported(*scene_slot(ctx, views[i].scene));
+    if (lanes) {
+        if ((rc = render_views_lanes(ctx, *V, n, fp64))) return rc;
+    } else {
+        e = launch_pathtrace_views(ctx, V->dArgs, V->dTileOff, n, V->dCounter, (int)tot, fp64);
+        if (e != hipSuccess) return hip_fail(ctx, e, "views launch");
+    }
     for (int i = 0; i < n; ++i) {
         V->total[i] += views[i].nSamples;
         V->postOut[i] = nullptr;

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "rc_internal.h"
+#include "rc_shade.h"
 #include "retrocam_hip.h"
 
 namespace rc {
@@ -244,6 +245,7 @@ int rc_destroy(rc_ctx *ctx) {
     free_images(ctx->img);
     free_sppm(ctx->sppm);
     free_wavefront(ctx->wf);
+    free_lane_buffers(ctx->lanes);
     free_net(ctx->net);
     dfree(ctx->counters);
     (void)hipStreamDestroy(ctx->stream);
@@ -425,8 +427,15 @@ static int render_common(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *
         return set_error(ctx, RC_ERR_LIMIT, "sample count overflow");
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_images(ctx, cam->width, cam->height))) return rc;
-    hipError_t e = ctx->ptMode == RC_PT_WAVEFRONT ? launch_wavefront(ctx, *cam, *p, n)
-                                                  : launch_pathtrace(ctx, *cam, *p, n);
+    hipError_t e;
+    if (ctx->ptMode == RC_PT_WAVEFRONT) {
+        e = launch_wavefront(ctx, *cam, *p, n);
+    } else if (ctx->ptMode == RC_PT_LANES && lanes_supported(ctx->scene)) {
+        PassArgs A = pass_args(ctx, *cam, *p, n);
+        e = launch_lanes(ctx, A, ctx->laneThresh, ctx->laneVariant);
+    } else {
+        e = launch_pathtrace(ctx, *cam, *p, n);
+    }
     if (e != hipSuccess) return hip_fail(ctx, e, "pathtrace launch");
     ctx->totalSamples += n;
     return RC_OK;
@@ -477,10 +486,33 @@ int rc_debug_counters(rc_ctx *ctx, int32_t enable, uint64_t *out, int32_t n) {
 
 int32_t rc_total_samples(const rc_ctx *ctx) { return ctx ? ctx->totalSamples : -1; }
 
+int rc_get_tuning(const rc_ctx *ctx, rc_tuning *out) {
+    if (!ctx || !out) return RC_ERR_INVALID_ARG;
+    out->laneThreshold = ctx->laneThresh;
+    out->laneVariant = ctx->laneVariant;
+    out->laneBufferMiB = ctx->laneBufferMiB;
+    return RC_OK;
+}
+
+int rc_set_tuning(rc_ctx *ctx, const rc_tuning *t) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (!t) return set_error(ctx, RC_ERR_INVALID_ARG, "tuning is NULL");
+    if (t->laneThreshold < 1 || t->laneThreshold > 64)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "laneThreshold must be in 1..64");
+    if (t->laneVariant < 0 || t->laneVariant >= RC_LANE_VARIANTS)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "laneVariant out of range");
+    if (t->laneBufferMiB < 64 || t->laneBufferMiB > 65536)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "laneBufferMiB must be in 64..65536");
+    ctx->laneThresh = t->laneThreshold;
+    ctx->laneVariant = t->laneVariant;
+    ctx->laneBufferMiB = t->laneBufferMiB;
+    return RC_OK;
+}
+
 int rc_set_pt_mode(rc_ctx *ctx, int32_t mode) {
     if (!ctx) return RC_ERR_INVALID_ARG;
-    if (mode != RC_PT_MEGAKERNEL && mode != RC_PT_WAVEFRONT)
-        return set_error(ctx, RC_ERR_INVALID_ARG, "mode must be RC_PT_MEGAKERNEL or RC_PT_WAVEFRONT");
+    if (mode != RC_PT_MEGAKERNEL && mode != RC_PT_WAVEFRONT && mode != RC_PT_LANES)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "mode must be RC_PT_LANES, RC_PT_MEGAKERNEL or RC_PT_WAVEFRONT");
     ctx->ptMode = mode;
     return RC_OK;
 }

@@ -91,6 +91,15 @@ struct Wavefront {
     int *hostCnt = nullptr;  // pinned
 };
 
+// Sample buffer and unit counters of the lane-scheduler path tracer
+// (rc_pathtrace_lanes.hip); grown on demand, kept for the context's lifetime.
+struct LaneBuffers {
+    float4 *samples = nullptr;
+    size_t sampleBytes = 0;
+    int *counters = nullptr;
+    int nCounters = 0;
+};
+
 struct Net;    // learned denoiser (rc_net.hip)
 struct Views;  // batched views (rc_batch.hip)
 
@@ -120,7 +129,13 @@ struct rc_ctx {
     int32_t accumMode = RC_ACCUM_FP32;
     rc::Sppm sppm;
     rc::Wavefront wf;
-    int32_t ptMode = RC_PT_MEGAKERNEL;
+    int32_t ptMode = RC_PT_LANES;
+    rc::LaneBuffers lanes;
+    int32_t laneThresh = 40;  // lanes waiting before a wave shades (rc_pathtrace_lanes.hip)
+    int32_t laneVariant = 4;  // lane-kernel variant (rc_pathtrace_lanes.hip)
+    // Upper bound of the lane scheduler's sample buffer (16 B per pixel and
+    // sample): a launch with more samples x pixels runs in chunks.
+    int32_t laneBufferMiB = 4096;
     rc::Net *net = nullptr;
     int32_t denoiser = RC_DENOISE_ATROUS;
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)

@@ -232,7 +232,7 @@ CamArgs cam_args(const rc_camera &cam) {
     return c;
 }
 
-hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples) {
+PassArgs pass_args(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples) {
     PassArgs A;
     A.S = scene_view(ctx->scene);
     A.cam = cam_args(cam);
@@ -249,6 +249,11 @@ hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_param
     A.primId = ctx->img.primId;
     A.gbufId = ctx->img.gbufId;
     A.counters = ctx->counters;
+    return A;
+}
+
+hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples) {
+    const PassArgs A = pass_args(ctx, cam, p, nSamples);
     dim3 grid((cam.width + 15) / 16, (cam.height + 15) / 16);
     const bool stats = ctx->countersEnabled && ctx->counters;
     const bool fp64 = ctx->accumMode == RC_ACCUM_FP64;

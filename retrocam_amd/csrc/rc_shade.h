@@ -5,6 +5,7 @@
 #pragma once
 
 #include "rc_device.h"
+#include "rc_internal.h"
 #include "retrocam_hip.h"
 
 namespace rc {
@@ -33,6 +34,26 @@ struct PassArgs {
     int *primId;
     int *gbufId;
     unsigned long long *counters;
+};
+
+// Lane-scheduler form (rc_pathtrace_lanes.hip): one chunk of a view's samples.
+// samples: [nSamples][nTilePx] float4, nTilePx = pixels of the view rounded up
+// to 8x8 tiles; counter: the view's unit counter, zeroed before the launch.
+struct LaneChunk {
+    float4 *samples;
+    int *counter;
+    int s0;        // first sample of the chunk, relative to PassArgs::frame0
+    int nSamples;  // samples in this chunk (0: the view has none here)
+    int nTilePx;
+};
+
+// Per-pixel frame-order accumulation of one chunk (lanes_reduce_kernel).
+struct ReduceArgs {
+    const float4 *samples;
+    int nTilePx, nSamples, frame0, width, height;
+    float4 *accum;
+    double4 *accum64;
+    float *variance;
 };
 
 __device__ __forceinline__ v3 sample_vndf(v3 wo, float alpha, float ux, float uy) {
@@ -264,9 +285,27 @@ __device__ __forceinline__ bool shade_vertex(const SceneView &S, int neeEnabled,
 }
 
 CamArgs cam_args(const rc_camera &cam);  // rc_pathtrace.hip
+// Launch arguments of nSamples passes from frame ctx->totalSamples on the
+// context's scene and images (rc_pathtrace.hip).
+PassArgs pass_args(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
 // Persistent launch over nViews views (rc_pathtrace.hip); views and tileOff in
 // device memory, counter a device int the launch resets first.
 hipError_t launch_pathtrace_views(rc_ctx *ctx, const PassArgs *views, const int *tileOff, int nViews, int *counter,
                                   int maxBlocks, bool fp64);
+
+// Lane-scheduler path tracer (rc_pathtrace_lanes.hip). Usable when the scene's
+// BVH is shallower than 30 levels (lanes_supported); same images as the
+// megakernel. launch_lanes runs A.nSamples samples in chunks of at most
+// lanes_chunk_samples (sample buffer <= rc_ctx::laneBufferMiB); launch_lanes_views
+// runs one chunk of many views.
+bool lanes_supported(const DeviceScene &D);
+int lanes_chunk_samples(int width, int height, int nSamples, long long capBytes);
+hipError_t ensure_lane_buffers(rc_ctx *ctx, size_t sampleBytes, int counters);
+void free_lane_buffers(LaneBuffers &L);
+hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A, int thresh, int variant);
+// zeroCounters > 0: first zero that many unit counters (all chunks' at once).
+hipError_t launch_lanes_views(rc_ctx *ctx, const PassArgs *dViews, LaneChunk *dChunks, const long long *dPixOff,
+                              int nViews, long long totalPx, long long totalUnits, int thresh, int variant,
+                              bool fp64, int zeroCounters);
 
 }  // namespace rc

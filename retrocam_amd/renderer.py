@@ -113,6 +113,23 @@ class Renderer:
     def set_pt_mode(self, mode: int):
         self.ctx.check(self._lib.rc_set_pt_mode(self.ctx.h, int(mode)))
 
+    TUNING_FIELDS = ("laneThreshold", "laneVariant", "laneBufferMiB")
+
+    def get_tuning(self) -> dict:
+        t = N.RcTuning()
+        self.ctx.check(self._lib.rc_get_tuning(self.ctx.h, C.byref(t)))
+        return {k: int(getattr(t, k)) for k in self.TUNING_FIELDS}
+
+    def set_tuning(self, **kw):
+        """Sets any of laneThreshold, laneVariant, laneBufferMiB (the rest stay)."""
+        cur = self.get_tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update({k: int(v) for k, v in kw.items()})
+        t = N.RcTuning(*(cur[k] for k in self.TUNING_FIELDS))
+        self.ctx.check(self._lib.rc_set_tuning(self.ctx.h, C.byref(t)))
+
     _DT = {N.RC_BUF_ACCUM: (np.float32, 4), N.RC_BUF_GBUFFER: (np.uint16, 4), N.RC_BUF_GALBEDO: (np.uint16, 4),
            N.RC_BUF_VARIANCE: (np.float32, 1), N.RC_BUF_PRIMARY_ID: (np.int32, 1),
            N.RC_BUF_GBUFFER_ID: (np.int32, 1), N.RC_BUF_ACCUM64: (np.float64, 4),

@@ -26,7 +26,8 @@ def rel_mse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 0.01)))
 
 
-MODES = [pytest.param(N.RC_PT_MEGAKERNEL, id="megakernel"), pytest.param(N.RC_PT_WAVEFRONT, id="wavefront")]
+MODES = [pytest.param(N.RC_PT_MEGAKERNEL, id="megakernel"), pytest.param(N.RC_PT_WAVEFRONT, id="wavefront"),
+         pytest.param(N.RC_PT_LANES, id="lanes")]
 
 
 def gpu_render(ctx, scene, cam, params, spp, batch=1, fp64=False, mode=N.RC_PT_MEGAKERNEL):
@@ -150,9 +151,51 @@ def test_batched_passes_equal_single_passes(gpu_ctx, cornell):
     a = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=1, mode=N.RC_PT_MEGAKERNEL)
     b = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=4, mode=N.RC_PT_WAVEFRONT)
     c = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=3, mode=N.RC_PT_MEGAKERNEL)
+    d = gpu_render(gpu_ctx, w.scene, cam, w.params, 6, batch=5, mode=N.RC_PT_LANES)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
         assert np.array_equal(a[k], c[k]), k
+        assert np.array_equal(a[k], d[k]), k
+
+
+def test_lane_tuning_changes_no_result(gpu_ctx, cornell):
+    # Threshold, kernel variant and sample-buffer bound only reschedule work:
+    # every setting gives the same image.
+    w = cornell
+    cam = CameraSpec(**{**w.camera.__dict__, "width": 100, "height": 60})
+    SceneUploader(gpu_ctx).upload(w.scene)
+    r = Renderer(gpu_ctx)
+    ref = gpu_render(gpu_ctx, w.scene, cam, w.params, 4, batch=4, mode=N.RC_PT_MEGAKERNEL)
+    for th in (1, 17, 64):
+        r.set_tuning(laneThreshold=th)
+        got = gpu_render(gpu_ctx, w.scene, cam, w.params, 4, batch=4, mode=N.RC_PT_LANES)
+        for k in ref:
+            assert np.array_equal(ref[k], got[k]), (th, k)
+    for v in range(N.RC_LANE_VARIANTS):
+        r.set_tuning(laneThreshold=40, laneVariant=v, laneBufferMiB=64)  # 64 MiB: several chunks
+        got = gpu_render(gpu_ctx, w.scene, cam, w.params, 4, batch=4, mode=N.RC_PT_LANES)
+        for k in ref:
+            assert np.array_equal(ref[k], got[k]), (v, k)
+    r.set_tuning(laneThreshold=40, laneVariant=4, laneBufferMiB=4096)
+    for bad in ({"laneThreshold": 0}, {"laneThreshold": 65}, {"laneVariant": N.RC_LANE_VARIANTS},
+                {"laneBufferMiB": 63}):
+        with pytest.raises(RenderError) as e:
+            r.set_tuning(**bad)
+        assert e.value.code == N.RC_ERR_INVALID_ARG
+
+
+def test_lanes_deep_bvh_falls_back(gpu_ctx):
+    # A BVH 80 levels deep exceeds the shadow walk's 30-entry cap: RC_PT_LANES
+    # must give the megakernel's image (it runs the megakernel for such scenes).
+    from test_oracle_core import chain_scene
+    sc = chain_scene(80, leaf_on_left=False)
+    cam = CameraSpec(eye=(0.0, 0.0, 1.0), forward=(0.0, 0.0, -1.0), right=(1.0, 0.0, 0.0), up=(0.0, 1.0, 0.0),
+                     width=24, height=24, pinhole=True)
+    p = PtParams(1, 1.0, 4)
+    a = gpu_render(gpu_ctx, sc, cam, p, 2, batch=2, mode=N.RC_PT_MEGAKERNEL)
+    b = gpu_render(gpu_ctx, sc, cam, p, 2, batch=2, mode=N.RC_PT_LANES)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
 
 
 @pytest.mark.parametrize("mode", MODES)

@@ -31,9 +31,11 @@ def requests():
 
 
 def single(scene, req, spp_list, fp64=False):
+    # reference: single-view megakernel renders in a context of their own
     with RenderContext(0) as ctx:
         SceneUploader(ctx).upload(scene)
         r = Renderer(ctx)
+        r.set_pt_mode(N.RC_PT_MEGAKERNEL)
         if fp64:
             r.set_accum_mode(N.RC_ACCUM_FP64)
         r.reset()
@@ -48,8 +50,13 @@ def single(scene, req, spp_list, fp64=False):
     return out
 
 
-def test_views_equal_single_renders(gpu_ctx):
+VIEW_MODES = [pytest.param(N.RC_PT_LANES, id="lanes"), pytest.param(N.RC_PT_MEGAKERNEL, id="megakernel")]
+
+
+@pytest.mark.parametrize("mode", VIEW_MODES)
+def test_views_equal_single_renders(mode, gpu_ctx):
     scenes, reqs = requests()
+    Renderer(gpu_ctx).set_pt_mode(mode)
     vb = ViewBatch(gpu_ctx)
     for k, sc in enumerate(scenes):
         vb.upload_scene(k, sc)
@@ -66,8 +73,10 @@ def test_views_equal_single_renders(gpu_ctx):
         assert np.array_equal(vb.read(i, N.RC_BUF_DISPLAY), ref["rgba8"]), i
 
 
-def test_views_fp64(gpu_ctx):
+@pytest.mark.parametrize("mode", VIEW_MODES)
+def test_views_fp64(mode, gpu_ctx):
     scenes, reqs = requests()
+    Renderer(gpu_ctx).set_pt_mode(mode)
     vb = ViewBatch(gpu_ctx)
     for k, sc in enumerate(scenes):
         vb.upload_scene(k, sc)
@@ -80,6 +89,7 @@ def test_views_fp64(gpu_ctx):
 
 def test_views_reject_bad_requests(gpu_ctx):
     scenes, reqs = requests()
+    Renderer(gpu_ctx).set_pt_mode(N.RC_PT_LANES)
     vb = ViewBatch(gpu_ctx)
     vb.upload_scene(0, scenes[0])
     with pytest.raises(RenderError):
