@@ -98,6 +98,8 @@ struct LaneBuffers {
     size_t sampleBytes = 0;
     int *counters = nullptr;
     int nCounters = 0;
+    float4 *cont = nullptr;  // 3 float4 per resident lane: shadow-ray continuation
+    size_t contLanes = 0;
 };
 
 struct Net;    // learned denoiser (rc_net.hip)

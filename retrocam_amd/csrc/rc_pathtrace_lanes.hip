@@ -27,6 +27,10 @@
 //
 // Per-lane arithmetic (raygen, walks, shading, NEE) is the shared code of
 // rc_device.h / rc_shade.h, so images are bit-identical to the megakernel's.
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "rc_device.h"
 #include "rc_internal.h"
 #include "rc_shade.h"
@@ -67,15 +71,36 @@ __device__ __forceinline__ void walk_begin(const SceneView &S, Walk &w, float tm
     w.tEntry = tr;
 }
 
+// Stack entries below NS live in LDS, deeper ones in the private spill array.
+// The LDS read is issued unconditionally (clamped index) and the spill read only
+// for deep entries. Deep entries are stored as one 64-bit word: loads of two
+// different types cannot be folded into one flat load through a selected
+// LDS-or-scratch address, so the common pop stays a ds_read.
+template <int NS>
+__device__ __forceinline__ int2 stack_pop(const StackRef &st, const unsigned long long *spill, int top) {
+    int2 e = st.lds[top < NS ? top : NS - 1][st.lane];
+    if (top >= NS) {
+        const unsigned long long v = spill[top - NS];
+        e = make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
+    }
+    return e;
+}
+
+template <int NS>
+__device__ __forceinline__ void stack_push(const StackRef &st, unsigned long long *spill, int top, int2 e) {
+    if (top < NS) st.lds[top][st.lane] = e;
+    else spill[top - NS] = (unsigned long long)(uint32_t)e.x | ((unsigned long long)(uint32_t)e.y << 32);
+}
+
 // One node (or one leaf's triangles) of the walk; returns true when the walk has
 // ended. anyHit: stop at the first triangle with t < ht (shadow rays).
 template <bool STATS, int NS>
-__device__ __forceinline__ bool walk_step(const SceneView &S, Walk &w, bool anyHit, const StackRef &st, int2 *spill,
+__device__ __forceinline__ bool walk_step(const SceneView &S, Walk &w, bool anyHit, const StackRef &st, unsigned long long *spill,
                                           Counters &C) {
     if (!w.have) {
         if (w.top == 0) return true;
         --w.top;
-        const int2 e = w.top < NS ? st.lds[w.top][st.lane] : spill[w.top - NS];
+        const int2 e = stack_pop<NS>(st, spill, w.top);
         w.ref = e.x;
         w.tEntry = __int_as_float(e.y);
         if (STATS) C.nodePops++;
@@ -113,8 +138,7 @@ __device__ __forceinline__ bool walk_step(const SceneView &S, Walk &w, bool anyH
             if (hl && hr) {
                 const bool nearLeft = tl < trr;
                 const int2 far = nearLeft ? make_int2(r.y, __float_as_int(trr)) : make_int2(r.x, __float_as_int(tl));
-                if (w.top < NS) st.lds[w.top][st.lane] = far;
-                else spill[w.top - NS] = far;
+                stack_push<NS>(st, spill, w.top, far);
                 ++w.top;
                 w.ref = nearLeft ? r.x : r.y;
                 w.tEntry = nearLeft ? tl : trr;
@@ -132,12 +156,12 @@ __device__ __forceinline__ bool walk_step(const SceneView &S, Walk &w, bool anyH
 // lanes at inner nodes each run one short body per step.
 template <bool STATS, int NS>
 __device__ __forceinline__ bool walk_step_tri(const SceneView &S, Walk &w, bool anyHit, const StackRef &st,
-                                              int2 *spill, Counters &C) {
+                                              unsigned long long *spill, Counters &C) {
     if (w.li >= w.le) {
         if (!w.have) {
             if (w.top == 0) return true;
             --w.top;
-            const int2 e = w.top < NS ? st.lds[w.top][st.lane] : spill[w.top - NS];
+            const int2 e = stack_pop<NS>(st, spill, w.top);
             w.ref = e.x;
             w.tEntry = __int_as_float(e.y);
             if (STATS) C.nodePops++;
@@ -160,8 +184,7 @@ __device__ __forceinline__ bool walk_step_tri(const SceneView &S, Walk &w, bool 
                         const bool nearLeft = tl < trr;
                         const int2 far =
                             nearLeft ? make_int2(r.y, __float_as_int(trr)) : make_int2(r.x, __float_as_int(tl));
-                        if (w.top < NS) st.lds[w.top][st.lane] = far;
-                        else spill[w.top - NS] = far;
+                        stack_push<NS>(st, spill, w.top, far);
                         ++w.top;
                         w.ref = nearLeft ? r.x : r.y;
                         w.tEntry = nearLeft ? tl : trr;
@@ -192,13 +215,23 @@ __device__ __forceinline__ bool walk_step_tri(const SceneView &S, Walk &w, bool 
     return w.li >= w.le && w.top == 0;
 }
 
+// Packed per-lane path flags: ray kind (2 bits), LCA channel (2), bounce (5),
+// lastDiffuse, and whether the path ends after the pending shadow ray.
+constexpr uint32_t F_LASTDIFF = 1u << 9, F_CONTEND = 1u << 10;
+__device__ __forceinline__ uint32_t f_pack(int kind, int ch, int bounce, bool lastDiffuse, bool contEnd) {
+    return (uint32_t)kind | ((uint32_t)ch << 2) | ((uint32_t)bounce << 4) | (lastDiffuse ? F_LASTDIFF : 0u) |
+           (contEnd ? F_CONTEND : 0u);
+}
+
 // All units of one view (K samples from frame frame0, every pixel; A gives the
 // scene, camera, switches and images): runs until the view's counter is
 // exhausted and every lane of the wave is done.
 // samples: [K][tilesX * tilesY * 64] float4 (r, g, b, 0) per unit.
+// cont: [3][gridDim.x * kBlock] float4, one slot per lane: the path's next ray
+// and NEE weight while its shadow ray is walked (kept out of registers).
 template <bool STATS, int NS, bool TRI>
 __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restrict__ samples, int *counter,
-                           int thresh, const StackRef &st, int2 *spill, Counters &C) {
+                           int thresh, float4 *__restrict__ cont, const StackRef &st, unsigned long long *spill, Counters &C) {
     const SceneView &S = A.S;
     const CamArgs &cam = A.cam;
     const int lane = threadIdx.x & 63;
@@ -207,18 +240,17 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
     const int nTilePx = tilesX * tilesY * 64;
     const int nUnits = nTilePx * K;  // < 2^30, checked by the host
     const unsigned long long below = (1ull << lane) - 1ull;
+    const size_t nLanes = (size_t)gridDim.x * kBlock;
+    float4 *const myCont = cont + (size_t)blockIdx.x * kBlock + threadIdx.x;
 
     // An exhausted view costs one load per wave, not an atomic.
     if (__atomic_load_n(counter, __ATOMIC_RELAXED) >= nUnits) return;
 
-    int phase = PH_FETCH, kind = RAY_PATH;
-    int px = 0, py = 0, q = 0, s = 0;
-    int ch = 0, bounce = 0, firstTri = -1;
+    int phase = PH_FETCH;
+    uint32_t flags = 0, pxy = 0;  // pxy: px | py << 16
+    int s = 0, firstTri = -1;
     uint32_t seed = 0;
     v3 thr = mk(1.0f, 1.0f, 1.0f), rad = mk(0.0f, 0.0f, 0.0f);
-    v3 nsW = mk(0.0f, 0.0f, 0.0f), cro = mk(0.0f, 0.0f, 0.0f), crd = mk(0.0f, 0.0f, 1.0f);
-    float sR = 0.0f, sG = 0.0f, sB = 0.0f;
-    bool lastDiffuse = false, contEnd = false;
     Walk w;
     w.ro = mk(0.0f, 0.0f, 0.0f);
     w.rd = mk(0.0f, 0.0f, 1.0f);
@@ -244,9 +276,9 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                     const int ts = u >> 6;  // tile * K + sample
                     const int tile = ts / K;
                     s = ts - tile * K;
-                    q = tile * 64 + within;
-                    px = (tile % tilesX) * 8 + (within & 7);
-                    py = (tile / tilesX) * 8 + (within >> 3);
+                    const int px = (tile % tilesX) * 8 + (within & 7);
+                    const int py = (tile / tilesX) * 8 + (within >> 3);
+                    pxy = (uint32_t)px | ((uint32_t)py << 16);
                     if (px < cam.width && py < cam.height) phase = PH_START;
                 }
             }
@@ -254,9 +286,14 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
 
         // (2) set-up of new units and processing of finished walks.
         if (phase == PH_START || phase == PH_SHADE) {
+            const int px = (int)(pxy & 0xffffu), py = (int)(pxy >> 16);
             const size_t pi = (size_t)py * (size_t)cam.width + (size_t)px;
             const int frame = frame0 + s;
+            const int kind = (int)(flags & 3u);
+            int ch = (int)((flags >> 2) & 3u), bounce = (int)((flags >> 4) & 31u);
+            bool lastDiffuse = (flags & F_LASTDIFF) != 0u, contEnd = (flags & F_CONTEND) != 0u;
             bool endPath = false, newPath = false, trace = false;
+            int nextKind = RAY_PATH;
             float tmax = 1e30f;
             if (phase == PH_START) {
                 ch = 0;
@@ -265,7 +302,7 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                     // G-buffer from the seed-99991 camera ray (pathtrace.comp:590-595, 643-646)
                     uint32_t gs = init_seed(px, py, 99991);
                     gen_ray(cam, px, py, gs, 0.0f, w.ro, w.rd);
-                    kind = RAY_GBUF;
+                    nextKind = RAY_GBUF;
                     trace = true;
                 } else {
                     newPath = true;
@@ -284,13 +321,16 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                 A.gbufId[pi] = w.htri;
                 newPath = true;
             } else if (kind == RAY_SHADOW) {
-                if (w.htri < 0) rad = rad + nsW;  // unoccluded: add the NEE sample (pathtrace.comp:573)
+                const float4 c0 = myCont[0], c1 = myCont[nLanes];
+                if (w.htri < 0) {
+                    const float wz = myCont[2 * nLanes].x;
+                    rad = rad + mk(c0.w, c1.w, wz);  // unoccluded: add the NEE sample (pathtrace.comp:573)
+                }
                 if (contEnd) {
                     endPath = true;
                 } else {
-                    w.ro = cro;
-                    w.rd = crd;
-                    kind = RAY_PATH;
+                    w.ro = mk(c0.x, c0.y, c0.z);
+                    w.rd = mk(c1.x, c1.y, c1.z);
                     trace = true;
                 }
             } else {
@@ -327,14 +367,14 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                     if (rr) {
                         endPath = true;
                     } else if (ns.shoot) {
-                        nsW = ns.weight;
-                        cro = w.ro;
-                        crd = w.rd;
+                        myCont[0] = make_float4(w.ro.x, w.ro.y, w.ro.z, ns.weight.x);
+                        myCont[nLanes] = make_float4(w.rd.x, w.rd.y, w.rd.z, ns.weight.y);
+                        myCont[2 * nLanes] = make_float4(ns.weight.z, 0.0f, 0.0f, 0.0f);
                         contEnd = last;
                         w.ro = ns.o;
                         w.rd = ns.d;
                         tmax = ns.maxDist;
-                        kind = RAY_SHADOW;
+                        nextKind = RAY_SHADOW;
                         trace = true;
                     } else if (last) {
                         endPath = true;
@@ -344,23 +384,30 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                 }
             }
             if (endPath) {
-                if (!lca) {
-                    sR = rad.x; sG = rad.y; sB = rad.z;
-                } else if (ch == 0) {
-                    sR = rad.x;
-                } else if (ch == 1) {
-                    sG = rad.y;
-                } else {
-                    sB = rad.z;
-                }
+                // channel radiance: the sample itself, or one LCA channel whose partial
+                // results wait in the sample's slot (pathtrace.comp:611-637)
+                const size_t slot = (size_t)s * (size_t)nTilePx +
+                                    (size_t)((((py >> 3) * tilesX + (px >> 3)) << 6) | ((py & 7) << 3) | (px & 7));
                 if (lca && ch < 2) {
+                    float4 v = make_float4(rad.x, 0.0f, 0.0f, 0.0f);
+                    if (ch == 1) {
+                        v = samples[slot];
+                        v.y = rad.y;
+                    }
+                    samples[slot] = v;
                     ++ch;
                     newPath = true;
                 } else {
+                    float sR = rad.x, sG = rad.y, sB = rad.z;
+                    if (lca) {
+                        const float4 v = samples[slot];
+                        sR = v.x;
+                        sG = v.y;
+                    }
                     // sample complete (pathtrace.comp:633-637); lanes_reduce accumulates it
                     if (isnan(sR) || isnan(sG) || isnan(sB) || isinf(sR) || isinf(sG) || isinf(sB))
                         sR = sG = sB = 0.0f;
-                    samples[(size_t)s * (size_t)nTilePx + (size_t)q] = make_float4(sR, sG, sB, 0.0f);
+                    samples[slot] = make_float4(sR, sG, sB, 0.0f);
                     if (s == K - 1) A.primId[pi] = firstTri;
                     phase = PH_FETCH;
                 }
@@ -378,7 +425,7 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                 rad = mk(0.0f, 0.0f, 0.0f);
                 bounce = 0;
                 lastDiffuse = false;
-                kind = RAY_PATH;
+                nextKind = RAY_PATH;
                 trace = true;
                 if (STATS) C.paths++;
             }
@@ -386,7 +433,7 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                 walk_begin(S, w, tmax);
                 phase = PH_WALK;
                 if (STATS) {
-                    if (kind == RAY_SHADOW) {
+                    if (nextKind == RAY_SHADOW) {
                         C.shadowRays++;
                     } else {
                         C.rays++;
@@ -394,14 +441,16 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                     }
                 }
             }
+            flags = f_pack(nextKind, ch, bounce, lastDiffuse, contEnd);
         }
         if (__ballot(phase != PH_DONE) == 0ull) break;
 
         // (3) walk until no lane walks or `thresh` lanes wait for shading / a unit.
+        const bool anyHit = (flags & 3u) == (uint32_t)RAY_SHADOW;
         while (true) {
             if (phase == PH_WALK) {
-                const bool done = TRI ? walk_step_tri<STATS, NS>(S, w, kind == RAY_SHADOW, st, spill, C)
-                                      : walk_step<STATS, NS>(S, w, kind == RAY_SHADOW, st, spill, C);
+                const bool done = TRI ? walk_step_tri<STATS, NS>(S, w, anyHit, st, spill, C)
+                                      : walk_step<STATS, NS>(S, w, anyHit, st, spill, C);
                 if (done) phase = PH_SHADE;
             }
             if (__ballot(phase == PH_WALK) == 0ull) break;
@@ -418,13 +467,13 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
 //   5: W 6, NS 13, TRI
 template <bool STATS, int W, int NS, bool TRI>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
-pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh) {
+pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh, float4 *cont) {
     __shared__ int2 stk[NS][kBlock];
     const StackRef st{stk, (int)threadIdx.x};
-    int2 spill[64 - NS];
+    unsigned long long spill[64 - NS];
     Counters C;
     if (STATS) C = Counters{};
-    lanes_view<STATS, NS, TRI>(A, A.frame0, A.nSamples, samples, counter, thresh, st, spill, C);
+    lanes_view<STATS, NS, TRI>(A, A.frame0, A.nSamples, samples, counter, thresh, cont, st, spill, C);
     if (STATS) wave_add_counters(A.counters, C);
 }
 
@@ -434,23 +483,23 @@ pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh) {
 template <bool STATS, int W, int NS, bool TRI>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
 pathtrace_lanes_views_kernel(const PassArgs *__restrict__ views, const LaneChunk *__restrict__ chunks, int nViews,
-                             int thresh, unsigned long long *counters) {
+                             int thresh, float4 *cont, unsigned long long *counters) {
     __shared__ int2 stk[NS][kBlock];
     const StackRef st{stk, (int)threadIdx.x};
-    int2 spill[64 - NS];
+    unsigned long long spill[64 - NS];
     Counters C;
     if (STATS) C = Counters{};
     for (int v = 0; v < nViews; ++v) {
         const LaneChunk &ck = chunks[v];
         if (ck.nSamples <= 0) continue;
-        lanes_view<STATS, NS, TRI>(views[v], views[v].frame0 + ck.s0, ck.nSamples, ck.samples, ck.counter, thresh, st,
-                              spill, C);
+        lanes_view<STATS, NS, TRI>(views[v], views[v].frame0 + ck.s0, ck.nSamples, ck.samples, ck.counter, thresh,
+                                   cont, st, spill, C);
     }
     if (STATS) wave_add_counters(counters, C);
 }
 
-using LaneKernel = void (*)(PassArgs, float4 *, int *, int);
-using LaneViewsKernel = void (*)(const PassArgs *, const LaneChunk *, int, int, unsigned long long *);
+using LaneKernel = void (*)(PassArgs, float4 *, int *, int, float4 *);
+using LaneViewsKernel = void (*)(const PassArgs *, const LaneChunk *, int, int, float4 *, unsigned long long *);
 
 #define RC_LANE_VARIANT_TABLE(K, S)                                              \
     switch (variant) {                                                           \
@@ -572,15 +621,28 @@ __global__ void fill_int(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+// Resident workgroups per CU x CUs of a kernel on the current device, queried
+// once per (kernel, device): the query costs host time that a short launch
+// would otherwise wait for.
+int resident_blocks(const void *kernel) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kernel, dev});
+    if (it != cache.end()) return it->second;
+    int occ = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ < 1) occ = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    cache[{kernel, dev}] = occ * cus;
+    return occ * cus;
+}
+
 // Workgroups of a persistent launch: as many as are resident at once, but no
 // more than the units need.
 int persistent_blocks(const void *kernel, int units) {
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ < 1) occ = 1;
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-    long long grid = (long long)occ * cus;
+    long long grid = resident_blocks(kernel);
     const long long need = ((long long)units + kBlock - 1) / kBlock;
     if (grid > need) grid = need;
     return grid < 1 ? 1 : (int)grid;
@@ -623,8 +685,26 @@ hipError_t ensure_lane_buffers(rc_ctx *ctx, size_t sampleBytes, int counters) {
 void free_lane_buffers(LaneBuffers &L) {
     if (L.samples) (void)hipFree(L.samples);
     if (L.counters) (void)hipFree(L.counters);
+    if (L.cont) (void)hipFree(L.cont);
     L = LaneBuffers{};
 }
+
+namespace {
+
+// Continuation slots for every lane a launch of `kernel` can have resident.
+hipError_t ensure_cont(rc_ctx *ctx, const void *kernel) {
+    LaneBuffers &L = ctx->lanes;
+    const size_t lanes = (size_t)resident_blocks(kernel) * kBlock;
+    if (lanes <= L.contLanes) return hipSuccess;
+    if (L.cont) (void)hipFree(L.cont);
+    L.cont = nullptr;
+    L.contLanes = 0;
+    hipError_t e = hipMalloc((void **)&L.cont, lanes * 3 * sizeof(float4));
+    if (e == hipSuccess) L.contLanes = lanes;
+    return e;
+}
+
+}  // namespace
 
 hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A0, int thresh, int variant) {
     const int W = A0.cam.width, H = A0.cam.height;
@@ -635,6 +715,8 @@ hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A0, int thresh, int variant
     const bool stats = ctx->countersEnabled && ctx->counters;
     const bool fp64 = ctx->accumMode == RC_ACCUM_FP64;
     const LaneKernel kernel = lane_kernel(variant, stats);
+    e = ensure_cont(ctx, (const void *)kernel);
+    if (e != hipSuccess) return e;
     for (int s0 = 0; s0 < A0.nSamples; s0 += kMax) {
         PassArgs A = A0;
         A.frame0 = A0.frame0 + s0;
@@ -642,7 +724,7 @@ hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A0, int thresh, int variant
         hipLaunchKernelGGL(fill_int, dim3(1), dim3(256), 0, ctx->stream, ctx->lanes.counters, 1, 0);
         const int grid = persistent_blocks((const void *)kernel, nTilePx * A.nSamples);
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, A, ctx->lanes.samples,
-                           ctx->lanes.counters, thresh);
+                           ctx->lanes.counters, thresh, ctx->lanes.cont);
         ReduceArgs R;
         R.samples = ctx->lanes.samples;
         R.nTilePx = nTilePx;
@@ -667,13 +749,15 @@ hipError_t launch_lanes_views(rc_ctx *ctx, const PassArgs *dViews, LaneChunk *dC
                               bool fp64, int zeroCounters) {
     const bool stats = ctx->countersEnabled && ctx->counters;
     const LaneViewsKernel kernel = lane_views_kernel(variant, stats);
+    hipError_t e = ensure_cont(ctx, (const void *)kernel);
+    if (e != hipSuccess) return e;
     if (zeroCounters > 0)
         hipLaunchKernelGGL(fill_int, dim3((zeroCounters + 255) / 256), dim3(256), 0, ctx->stream, ctx->lanes.counters,
                            zeroCounters, 0);
     const int grid = persistent_blocks((const void *)kernel,
                                        totalUnits > (1ll << 30) ? (1 << 30) : (int)totalUnits);
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, dViews, dChunks, nViews, thresh,
-                       ctx->counters);
+                       ctx->lanes.cont, ctx->counters);
     const dim3 rg((unsigned)((totalPx + 255) / 256));
     if (fp64) hipLaunchKernelGGL(lanes_reduce_views_kernel<true>, rg, dim3(256), 0, ctx->stream, dViews, dChunks,
                                  dPixOff, nViews);
