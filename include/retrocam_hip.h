@@ -334,6 +334,39 @@ int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t 
 /* Which denoiser rc_post runs when rc_post_params.atrousEnabled is set. */
 int rc_set_denoiser(rc_ctx *ctx, int32_t mode);
 
+/* ---- captured frames (no counterpart in the reference) ------------------- */
+/* RenderPipeline.accumulateFrame (src/main/java/com/retrocam/export/RenderPipeline.java:209-239)
+ * issues the same chain of dispatches every frame: reset, spp x {photon trace,
+ * path trace, gather, radius update}, post chain, display. These calls record
+ * such a chain once into a HIP graph and replay it with one launch.
+ *
+ * Between rc_frame_capture_begin and rc_frame_capture_end, the device work of
+ * rc_reset, rc_render_pass(es), rc_sppm_trace / _gather / _update_radius,
+ * rc_post, rc_views_render and rc_views_post is recorded instead of run (host
+ * state such as the sample count advances as usual). The recording must begin
+ * with rc_reset before any single-view rendering, and with RC_VIEWS_RESET in
+ * its first rc_views_render, so that every replay recomputes the frame.
+ * Nothing recorded may allocate, wait or copy to the host: run the same frame
+ * once before capturing it (buffers, network workspace and view tables then
+ * exist); calls that would do so return RC_ERR_STATE (or RC_ERR_HIP for a
+ * buffer that would have to grow) and end nothing - call rc_frame_capture_end
+ * to drop the recording. rc_read, rc_sync, uploads, rc_net_* and the wavefront
+ * path tracer are refused while capturing.
+ *
+ * rc_frame_replay launches the recorded frame on the context's stream and
+ * restores the host state to its end (sample count, SPPM radius and
+ * iteration, view sample counts). t, when not NULL, replaces the temporal
+ * inputs of every recorded post chain (video time, AGC gain, frame index);
+ * NULL replays them as recorded. A frame that renders views reads the view
+ * tables of the last rc_views_render; replaying while other views' tables are
+ * on the device returns RC_ERR_STATE (rendering the recorded views again
+ * restores them). One recorded frame per context; capturing again or
+ * rc_frame_release drops it. */
+int rc_frame_capture_begin(rc_ctx *ctx);
+int rc_frame_capture_end(rc_ctx *ctx);
+int rc_frame_replay(rc_ctx *ctx, const rc_temporal *t);
+int rc_frame_release(rc_ctx *ctx);
+
 /* ---- tuning (no counterpart in the reference) --------------------------- */
 /* Launch parameters that change speed, never results. */
 #define RC_LANE_VARIANTS 6

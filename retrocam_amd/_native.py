@@ -157,6 +157,10 @@ HIP_SYMBOLS = {
     "rc_net_grad": (C.c_int, [_VP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP, _FP]),
     "rc_net_forward": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP]),
     "rc_set_denoiser": (C.c_int, [_VP, C.c_int32]),
+    "rc_frame_capture_begin": (C.c_int, [_VP]),
+    "rc_frame_capture_end": (C.c_int, [_VP]),
+    "rc_frame_replay": (C.c_int, [_VP, C.POINTER(RcTemporal)]),
+    "rc_frame_release": (C.c_int, [_VP]),
     "rc_get_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_set_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),

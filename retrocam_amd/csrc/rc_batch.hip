@@ -9,6 +9,7 @@
 // work (samples x tiles, heaviest first) and the prefix sum of their 16x16
 // tile counts, which the persistent kernel walks with an atomic tile counter.
 #include <algorithm>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -41,6 +42,8 @@ struct Views {
     int chunkCap = 0;
     LaneChunk *dChunks = nullptr, *hChunks = nullptr;
     long long *dPixOff = nullptr, *hPixOff = nullptr;
+    int uploadedN = -1;              // table sizes of the last upload
+    long long uploadedEntries = -1;
 };
 
 namespace {
@@ -68,6 +71,8 @@ void free_lane_tables(Views &v) {
 
 void free_tables(Views &v) {
     free_lane_tables(v);
+    v.uploadedN = -1;
+    v.uploadedEntries = -1;
     vfree(v.dArgs); vfree(v.dTileOff); vfree(v.dCounter);
     if (v.hArgs) (void)hipHostFree(v.hArgs);
     if (v.hTileOff) (void)hipHostFree(v.hTileOff);
@@ -103,6 +108,7 @@ int layout(rc_ctx *ctx, Views &V, const rc_view *views, int n, bool fp64, bool &
     changed = V.n != n || V.fp64 != fp64;
     for (int i = 0; i < n && !changed; ++i) changed = V.w[i] != views[i].cam.width || V.h[i] != views[i].cam.height;
     if (!changed) return RC_OK;
+    if (int rc = capture_block(ctx, "a new set of view sizes")) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
     V.w.resize(n); V.h.resize(n); V.total.assign(n, 0); V.pixOff.resize(n); V.postOut.assign(n, nullptr);
@@ -114,6 +120,7 @@ int layout(rc_ctx *ctx, Views &V, const rc_view *views, int n, bool fp64, bool &
         px += (size_t)V.w[i] * V.h[i];
     }
     if (px > V.pixCap || (fp64 && !V.accum64)) {
+        ++ctx->freeGen;
         free_arenas(V);
         if (e == hipSuccess) e = valloc(V.accum, px);
         if (e == hipSuccess && fp64) e = valloc(V.accum64, px);
@@ -133,6 +140,7 @@ int layout(rc_ctx *ctx, Views &V, const rc_view *views, int n, bool fp64, bool &
         V.pixCap = px;
     }
     if (n > V.viewCap) {
+        ++ctx->freeGen;
         free_tables(V);
         if (e == hipSuccess) e = valloc(V.dArgs, n);
         if (e == hipSuccess) e = valloc(V.dTileOff, n + 1);
@@ -151,37 +159,102 @@ int layout(rc_ctx *ctx, Views &V, const rc_view *views, int n, bool fp64, bool &
     return RC_OK;
 }
 
-// The views of a batch traced by the lane scheduler, in chunks of samples
-// that keep the sample buffer within rc_ctx::laneBufferMiB. hArgs[0..n) is
-// the sorted table already copied to dArgs.
-int render_views_lanes(rc_ctx *ctx, Views &V, int n, bool fp64) {
+// Host tables of one batch: the per-view launch arguments (sorted heaviest
+// first), the tile prefix sums of the megakernel form and, for the lane
+// scheduler, the chunk x view table and the pixel prefix sums.
+struct ViewTables {
+    std::vector<PassArgs> args;
+    std::vector<int> tileOff;
+    bool lanes = false;
+    int nChunks = 0;
+    std::vector<LaneChunk> chunks;
+    std::vector<long long> pixOff;
+    std::vector<long long> units;  // per chunk
+    long long sampleBytes = 0;
+};
+
+// Lane-scheduler chunk plan: the largest chunk K (samples per view and launch)
+// whose sample buffer fits rc_ctx::laneBufferMiB.
+int plan_lanes(rc_ctx *ctx, ViewTables &T) {
+    const int n = (int)T.args.size();
     std::vector<long long> tilePx(n);
     int maxS = 0;
     for (int k = 0; k < n; ++k) {
-        const CamArgs &c = V.hArgs[k].cam;
+        const CamArgs &c = T.args[k].cam;
         tilePx[k] = (long long)((c.width + 7) / 8) * ((c.height + 7) / 8) * 64;
-        maxS = std::max(maxS, V.hArgs[k].nSamples);
+        maxS = std::max(maxS, T.args[k].nSamples);
     }
-    // largest chunk K whose buffer fits, then whole chunks of K samples per view
     auto bytesFor = [&](long long K) {
         long long b = 0;
-        for (int k = 0; k < n; ++k) b += std::min<long long>(K, V.hArgs[k].nSamples) * tilePx[k] * 16;
+        for (int k = 0; k < n; ++k) b += std::min<long long>(K, T.args[k].nSamples) * tilePx[k] * 16;
         return b;
     };
-    long long K = maxS;
     const long long cap = (long long)ctx->laneBufferMiB << 20;
+    long long K = maxS;
     while (K > 1 && bytesFor(K) > cap) K = (K + 1) / 2;
     for (int k = 0; k < n; ++k)
         while (K > 1 && tilePx[k] * K >= (1ll << 30)) K = (K + 1) / 2;
-    if (tilePx[0] * K >= (1ll << 30)) return set_error(ctx, RC_ERR_LIMIT, "views: view too large");
-    const int nChunks = (int)((maxS + K - 1) / K);
-    const long long entries = (long long)nChunks * n;
+    for (int k = 0; k < n; ++k)
+        if (tilePx[k] * K >= (1ll << 30)) return set_error(ctx, RC_ERR_LIMIT, "views: view too large");
+    T.nChunks = (int)((maxS + K - 1) / K);
+    const long long entries = (long long)T.nChunks * n;
     if (entries > (1 << 24)) return set_error(ctx, RC_ERR_LIMIT, "views: too many chunks");
-    hipError_t e = ensure_lane_buffers(ctx, (size_t)bytesFor(K), (int)entries);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(lane buffers)");
-    if (entries > V.chunkCap || !V.dPixOff || n + 1 > V.viewCap + 1) {
+    T.sampleBytes = bytesFor(K);
+    hipError_t e = ensure_lane_buffers(ctx, (size_t)T.sampleBytes, (int)entries);
+    if (e != hipSuccess) return hip_fail(ctx, e, "lane buffers");
+    T.chunks.assign((size_t)entries, LaneChunk{});
+    T.pixOff.assign(n + 1, 0);
+    T.units.assign(T.nChunks, 0);
+    long long px = 0, off = 0;
+    for (int k = 0; k < n; ++k) {
+        T.pixOff[k] = px;
+        px += (long long)T.args[k].cam.width * T.args[k].cam.height;
+        for (int c = 0; c < T.nChunks; ++c) {
+            LaneChunk &L = T.chunks[(size_t)c * n + k];
+            const long long s0 = (long long)c * K;
+            const long long ns = std::max(0ll, std::min<long long>(K, T.args[k].nSamples - s0));
+            L.samples = ctx->lanes.samples + off;
+            L.counter = ctx->lanes.counters + (size_t)c * n + k;
+            L.s0 = (int)s0;
+            L.nSamples = (int)ns;
+            L.nTilePx = (int)tilePx[k];
+            T.units[c] += ns * tilePx[k];
+        }
+        off += std::min<long long>(K, T.args[k].nSamples) * tilePx[k];
+    }
+    T.pixOff[n] = px;
+    return RC_OK;
+}
+
+// Uploads the tables (eager calls), or, while a frame is being captured,
+// checks that they equal the tables already on the device: a recorded frame
+// may not copy host memory, so it reuses the last upload.
+int upload_tables(rc_ctx *ctx, Views &V, const ViewTables &T) {
+    const int n = (int)T.args.size();
+    const size_t entries = T.chunks.size();
+    if (ctx->capturing) {
+        bool same = V.uploadedN == n && V.uploadedEntries == (long long)entries &&
+                    std::memcmp(V.hArgs, T.args.data(), sizeof(PassArgs) * n) == 0 &&
+                    std::memcmp(V.hTileOff, T.tileOff.data(), sizeof(int) * (n + 1)) == 0;
+        if (same && T.lanes)
+            same = V.hChunks && V.hPixOff &&
+                   std::memcmp(V.hChunks, T.chunks.data(), sizeof(LaneChunk) * entries) == 0 &&
+                   std::memcmp(V.hPixOff, T.pixOff.data(), sizeof(long long) * (n + 1)) == 0;
+        if (!same)
+            return set_error(ctx, RC_ERR_STATE,
+                             "captured rc_views_render: these views differ from the last rendered ones; render them "
+                             "once before capturing");
+        ctx->graphUsesViews = 1;
+        ctx->graphViewTablesSig = ctx->viewTablesSig;
+        return RC_OK;
+    }
+    // The pinned staging may still be read by a previous launch's copy.
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    if (T.lanes && ((long long)entries > V.chunkCap || !V.dPixOff)) {
+        ++ctx->freeGen;
         free_lane_tables(V);
-        e = valloc(V.dChunks, (size_t)entries);
+        e = valloc(V.dChunks, entries);
         if (e == hipSuccess) e = valloc(V.dPixOff, (size_t)V.viewCap + 1);
         if (e == hipSuccess) e = hipHostMalloc((void **)&V.hChunks, sizeof(LaneChunk) * entries, hipHostMallocDefault);
         if (e == hipSuccess)
@@ -192,34 +265,32 @@ int render_views_lanes(rc_ctx *ctx, Views &V, int n, bool fp64) {
         }
         V.chunkCap = (int)entries;
     }
-    long long px = 0, off = 0;
-    for (int k = 0; k < n; ++k) {
-        V.hPixOff[k] = px;
-        px += (long long)V.hArgs[k].cam.width * V.hArgs[k].cam.height;
-        for (int c = 0; c < nChunks; ++c) {
-            LaneChunk &L = V.hChunks[(size_t)c * n + k];
-            const long long s0 = (long long)c * K;
-            const long long ns = std::max(0ll, std::min<long long>(K, V.hArgs[k].nSamples - s0));
-            L.samples = ctx->lanes.samples + off;
-            L.counter = ctx->lanes.counters + (size_t)c * n + k;
-            L.s0 = (int)s0;
-            L.nSamples = (int)ns;
-            L.nTilePx = (int)tilePx[k];
-        }
-        off += std::min<long long>(K, V.hArgs[k].nSamples) * tilePx[k];
-    }
-    V.hPixOff[n] = px;
-    e = hipMemcpyAsync(V.dChunks, V.hChunks, sizeof(LaneChunk) * entries, hipMemcpyHostToDevice, ctx->stream);
+    std::memcpy(V.hArgs, T.args.data(), sizeof(PassArgs) * n);
+    std::memcpy(V.hTileOff, T.tileOff.data(), sizeof(int) * (n + 1));
+    e = hipMemcpyAsync(V.dArgs, V.hArgs, sizeof(PassArgs) * n, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(V.dPixOff, V.hPixOff, sizeof(long long) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "view lane tables upload");
-    for (int c = 0; c < nChunks; ++c) {
-        long long units = 0;
-        for (int k = 0; k < n; ++k) units += (long long)V.hChunks[(size_t)c * n + k].nSamples * tilePx[k];
-        e = launch_lanes_views(ctx, V.dArgs, V.dChunks + (size_t)c * n, V.dPixOff, n, px, units, ctx->laneThresh,
-                               ctx->laneVariant, fp64, c == 0 ? (int)entries : 0);
-        if (e != hipSuccess) return hip_fail(ctx, e, "views lane launch");
+        e = hipMemcpyAsync(V.dTileOff, V.hTileOff, sizeof(int) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && T.lanes) {
+        std::memcpy(V.hChunks, T.chunks.data(), sizeof(LaneChunk) * entries);
+        std::memcpy(V.hPixOff, T.pixOff.data(), sizeof(long long) * (n + 1));
+        e = hipMemcpyAsync(V.dChunks, V.hChunks, sizeof(LaneChunk) * entries, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(V.dPixOff, V.hPixOff, sizeof(long long) * (n + 1), hipMemcpyHostToDevice,
+                               ctx->stream);
     }
+    if (e != hipSuccess) return hip_fail(ctx, e, "views table upload");
+    V.uploadedN = n;
+    V.uploadedEntries = (long long)entries;
+    uint64_t sig = 1469598103934665603ull;  // FNV-1a over the uploaded bytes
+    auto mix = [&sig](const void *p, size_t bytes) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t i = 0; i < bytes; ++i) sig = (sig ^ c[i]) * 1099511628211ull;
+    };
+    mix(T.args.data(), sizeof(PassArgs) * n);
+    mix(T.tileOff.data(), sizeof(int) * (n + 1));
+    mix(T.chunks.data(), sizeof(LaneChunk) * entries);
+    mix(T.pixOff.data(), sizeof(long long) * T.pixOff.size());
+    ctx->viewTablesSig = sig;
     return RC_OK;
 }
 
@@ -229,6 +300,14 @@ Views *get_views(rc_ctx *ctx) {
 }
 
 }  // namespace
+
+std::vector<int> views_totals(const rc_ctx *ctx) {
+    return ctx->views ? ctx->views->total : std::vector<int>();
+}
+
+void views_set_totals(rc_ctx *ctx, const std::vector<int> &totals) {
+    if (ctx->views && ctx->views->total.size() == totals.size()) ctx->views->total = totals;
+}
 
 void free_views(rc_ctx *ctx) {
     if (!ctx->views) return;
@@ -250,33 +329,40 @@ int rc_views_render(rc_ctx *ctx, const rc_view *views, int32_t n, int32_t flags)
     int rc;
     for (int i = 0; i < n; ++i)
         if ((rc = validate_view(ctx, views[i]))) return rc;
+    if (ctx->capturing) {
+        if (flags & RC_VIEWS_RESET) ctx->captureViewsResetSeen = 1;
+        if (!ctx->captureViewsResetSeen)
+            return set_error(ctx, RC_ERR_STATE,
+                             "a captured frame's first rc_views_render must reset the views (RC_VIEWS_RESET)");
+    }
     (void)hipSetDevice(ctx->device);
     Views *V = get_views(ctx);
     if (!V) return set_error(ctx, RC_ERR_OOM, "host allocation failed");
     const bool fp64 = (flags & RC_VIEWS_FP64) != 0;
     bool changed = false;
     if ((rc = layout(ctx, *V, views, n, fp64, changed))) return rc;
-    if (flags & RC_VIEWS_RESET) V->total.assign(n, 0);
-    // The pinned staging table may still be read by a previous launch's copy.
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    std::vector<int> total = V->total;
+    if (flags & RC_VIEWS_RESET) total.assign(n, 0);
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     auto tiles = [&](int i) { return ((V->w[i] + 15) / 16) * ((V->h[i] + 15) / 16); };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return (long long)views[a].nSamples * tiles(a) > (long long)views[b].nSamples * tiles(b);
     });
+    ViewTables T;
+    T.args.assign(n, PassArgs{});
+    T.tileOff.assign(n + 1, 0);
     long long tot = 0;
     for (int k = 0; k < n; ++k) {
         const int i = order[k];
         const rc_view &v = views[i];
-        PassArgs &A = V->hArgs[k];
+        PassArgs &A = T.args[k];
         A.S = scene_view(*scene_slot(ctx, v.scene));
         A.cam = cam_args(v.cam);
         A.nee = v.params.neeEnabled;
         A.neeClamp = v.params.neeFireflyClamp;
         A.maxBounces = v.params.maxBounces;
-        A.frame0 = V->total[i];
+        A.frame0 = total[i];
         A.nSamples = v.nSamples;
         const size_t o = V->pixOff[i];
         A.accum = V->accum + o;
@@ -287,25 +373,26 @@ int rc_views_render(rc_ctx *ctx, const rc_view *views, int32_t n, int32_t flags)
         A.primId = V->primId + o;
         A.gbufId = V->gbufId + o;
         A.counters = ctx->counters;
-        V->hTileOff[k] = (int)tot;
+        T.tileOff[k] = (int)tot;
         tot += tiles(i);
     }
     if (tot >= (1ll << 30)) return set_error(ctx, RC_ERR_LIMIT, "views: too many tiles");
-    V->hTileOff[n] = (int)tot;
-    e = hipMemcpyAsync(V->dArgs, V->hArgs, sizeof(PassArgs) * n, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(V->dTileOff, V->hTileOff, sizeof(int) * (n + 1), hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "views table upload");
-    bool lanes = ctx->ptMode == RC_PT_LANES;
-    for (int i = 0; i < n && lanes; ++i) lanes = lanes_supported(*scene_slot(ctx, views[i].scene));
-    if (lanes) {
-        if ((rc = render_views_lanes(ctx, *V, n, fp64))) return rc;
+    T.tileOff[n] = (int)tot;
+    T.lanes = ctx->ptMode == RC_PT_LANES;
+    for (int i = 0; i < n && T.lanes; ++i) T.lanes = lanes_supported(*scene_slot(ctx, views[i].scene));
+    if (T.lanes && (rc = plan_lanes(ctx, T))) return rc;
+    if ((rc = upload_tables(ctx, *V, T))) return rc;
+    hipError_t e = hipSuccess;
+    if (T.lanes) {
+        for (int c = 0; c < T.nChunks && e == hipSuccess; ++c)
+            e = launch_lanes_views(ctx, V->dArgs, V->dChunks + (size_t)c * n, V->dPixOff, n, T.pixOff[n], T.units[c],
+                                   ctx->laneThresh, ctx->laneVariant, fp64, c == 0 ? (int)T.chunks.size() : 0);
     } else {
         e = launch_pathtrace_views(ctx, V->dArgs, V->dTileOff, n, V->dCounter, (int)tot, fp64);
-        if (e != hipSuccess) return hip_fail(ctx, e, "views launch");
     }
+    if (e != hipSuccess) return hip_fail(ctx, e, "views launch");
     for (int i = 0; i < n; ++i) {
-        V->total[i] += views[i].nSamples;
+        V->total[i] = total[i] + views[i].nSamples;
         V->postOut[i] = nullptr;
     }
     return RC_OK;
@@ -356,6 +443,7 @@ size_t rc_views_buffer_bytes(const rc_ctx *ctx, int32_t view, int32_t which) {
 
 int rc_views_read(rc_ctx *ctx, int32_t view, int32_t which, void *dst, size_t bytes) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc = capture_block(ctx, "rc_views_read")) return rc;
     Views *V = ctx->views;
     if (!V || view < 0 || view >= V->n) return set_error(ctx, RC_ERR_STATE, "view index out of range");
     const size_t need = rc_views_buffer_bytes(ctx, view, which);

@@ -27,6 +27,44 @@ int hip_fail(rc_ctx *ctx, hipError_t e, const char *what) {
     return set_error(ctx, e == hipErrorOutOfMemory ? RC_ERR_OOM : RC_ERR_HIP, buf);
 }
 
+int capture_block(rc_ctx *ctx, const char *what) {
+    if (!ctx->capturing) return RC_OK;
+    return set_error(ctx, RC_ERR_STATE,
+                     std::string(what) + " is not allowed while a frame is being captured (it would allocate, wait "
+                                         "for the device or copy to the host); run the frame once before capturing it");
+}
+
+const rc_temporal *temporal_slot(rc_ctx *ctx, const rc_temporal &t) {
+    if (!ctx->capturing || ctx->nTemporal >= kMaxTemporalSlots) return nullptr;
+    ctx->hTemporal[ctx->nTemporal] = t;
+    return ctx->dTemporal + ctx->nTemporal++;
+}
+
+// Recorded single-view work must follow an rc_reset of the same capture, so that
+// every replay recomputes the frame instead of adding to the previous one.
+static int capture_needs_reset(rc_ctx *ctx) {
+    if (!ctx->capturing || ctx->captureResetSeen) return RC_OK;
+    return set_error(ctx, RC_ERR_STATE, "a captured frame must begin with rc_reset before rendering into it");
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_temporal_kernel(rc_temporal *p, int n, rc_temporal t) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = t;
+}
+
+// Zero fill by kernel (a graph-capturable node; no memset nodes in captured frames).
+static hipError_t zero_fill(hipStream_t s, void *p, size_t bytes) {
+    const size_t n = bytes / 4;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (uint32_t *)p, n, 0u);
+    return hipGetLastError();
+}
+
 template <typename T>
 static void dfree(T *&p) {
     if (p) (void)hipFree((void *)p);
@@ -53,8 +91,10 @@ static int dalloc(rc_ctx *ctx, T *&p, size_t count, const char *what);
 static int ensure_post(rc_ctx *ctx, int w, int h) {
     Images &im = ctx->img;
     if (im.postW == w && im.postH == h && im.postA) return RC_OK;
+    if (int rc = capture_block(ctx, "allocating the post-chain images")) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    ++ctx->freeGen;
     dfree(im.postA); dfree(im.postB); dfree(im.postIn); dfree(im.display);
     im.postOut = nullptr;
     im.postW = im.postH = 0;
@@ -74,8 +114,10 @@ static int ensure_post(rc_ctx *ctx, int w, int h) {
 int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf) {
     const int w = io.w, h = io.h;
     float4 *a = io.a, *b = io.b;
+    const rc_temporal *dt = temporal_slot(ctx, t);
+    if (ctx->capturing && !dt) return set_error(ctx, RC_ERR_LIMIT, "captured frame: too many post chains");
     hipError_t e = launch_post_pass(ctx->stream, 0, io.src, a, w, h, p, t, io.gbuf, io.galbedo, io.variance,
-                                    io.total, 0, nullptr);
+                                    io.total, 0, nullptr, dt);
     if (e != hipSuccess) return hip_fail(ctx, e, "post p00 launch");
     bool exceeded = p.atrousMaxSpp > 0 && io.total > p.atrousMaxSpp;
     if (p.atrousEnabled && withGbuf && !exceeded && ctx->denoiser == RC_DENOISE_NET && net_ready(ctx)) {
@@ -86,18 +128,18 @@ int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_tempor
         int np = p.atrousIterations < 1 ? 1 : (p.atrousIterations > 5 ? 5 : p.atrousIterations);
         for (int i = 0; i < np; ++i) {  // AtrousDenoiser.denoise (AtrousDenoiser.java:60-125)
             e = launch_post_pass(ctx->stream, 100, a, b, w, h, p, t, io.gbuf, io.galbedo, io.variance, io.total,
-                                 1 << i, nullptr);
+                                 1 << i, nullptr, dt);
             if (e != hipSuccess) return hip_fail(ctx, e, "post a-trous launch");
             float4 *tmp = a; a = b; b = tmp;
         }
     }
     for (int k = 1; k <= 16; ++k) {
         if (k == 5 || k == 6 || !p.enable[k]) continue;  // p05/p06 pass through (PostProcessStack.java:184)
-        e = launch_post_pass(ctx->stream, k, a, b, w, h, p, t, nullptr, nullptr, nullptr, io.total, 0, nullptr);
+        e = launch_post_pass(ctx->stream, k, a, b, w, h, p, t, nullptr, nullptr, nullptr, io.total, 0, nullptr, dt);
         if (e != hipSuccess) return hip_fail(ctx, e, "post pass launch");
         float4 *tmp = a; a = b; b = tmp;
     }
-    e = launch_post_pass(ctx->stream, 200, a, nullptr, w, h, p, t, nullptr, nullptr, nullptr, 1, 0, io.display);
+    e = launch_post_pass(ctx->stream, 200, a, nullptr, w, h, p, t, nullptr, nullptr, nullptr, 1, 0, io.display, dt);
     if (e != hipSuccess) return hip_fail(ctx, e, "display launch");
     io.out = a;
     return RC_OK;
@@ -149,8 +191,10 @@ static int ensure_images(rc_ctx *ctx, int w, int h) {
     Images &im = ctx->img;
     bool need64 = ctx->accumMode == RC_ACCUM_FP64;
     if (im.width == w && im.height == h && (!need64 || im.accum64)) return RC_OK;
+    if (int rc = capture_block(ctx, "allocating images for a new resolution")) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    ++ctx->freeGen;
     free_images(im);
     size_t n = (size_t)w * (size_t)h;
     int rc;
@@ -231,6 +275,16 @@ int rc_create(int device, rc_ctx **out) {
     }
     ctx->counters = (unsigned long long *)c;
     (void)hipMemsetAsync(ctx->counters, 0, 64 * sizeof(unsigned long long), ctx->stream);
+    e = hipMalloc((void **)&ctx->dTemporal, sizeof(rc_temporal) * kMaxTemporalSlots);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&ctx->hTemporal, sizeof(rc_temporal) * kMaxTemporalSlots, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipFree(ctx->counters);
+        if (ctx->dTemporal) (void)hipFree(ctx->dTemporal);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_error(nullptr, RC_ERR_OOM, "allocating the temporal-input slots failed");
+    }
     *out = ctx;
     return RC_OK;
 }
@@ -238,7 +292,17 @@ int rc_create(int device, rc_ctx **out) {
 int rc_destroy(rc_ctx *ctx) {
     if (!ctx) return RC_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
+    if (ctx->capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        ctx->capturing = 0;
+    }
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->frameExec) (void)hipGraphExecDestroy(ctx->frameExec);
+    if (ctx->frameGraph) (void)hipGraphDestroy(ctx->frameGraph);
+    if (ctx->dTemporal) (void)hipFree(ctx->dTemporal);
+    if (ctx->hTemporal) (void)hipHostFree(ctx->hTemporal);
     free_scene(ctx->scene);
     for (auto &d : ctx->extraScenes) free_scene(d);
     free_views(ctx);
@@ -259,6 +323,7 @@ void *rc_stream(rc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int rc_sync(rc_ctx *ctx) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc = capture_block(ctx, "rc_sync")) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
     return RC_OK;
@@ -275,6 +340,7 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
         return set_error(ctx, RC_ERR_INVALID_ARG, "NULL scene stream");
     if (nTris >= (1 << 26))
         return set_error(ctx, RC_ERR_LIMIT, "nTris must be < 2^26");
+    if (int rc = capture_block(ctx, "uploading a scene")) return rc;
     (void)hipSetDevice(ctx->device);
 
     // --- validate the BVH: children in range, strictly after the parent, each
@@ -361,6 +427,7 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
 
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    ++ctx->freeGen;
     free_scene(D);
     int rc;
     if ((rc = dalloc(ctx, D.inner, inner.size() ? inner.size() : 4, "hipMalloc(nodes)"))) return rc;
@@ -425,6 +492,9 @@ static int render_common(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *
     if (!ctx->scene.valid) return set_error(ctx, RC_ERR_STATE, "no scene uploaded");
     if ((int64_t)ctx->totalSamples + n > 0x7fffffff / 8)
         return set_error(ctx, RC_ERR_LIMIT, "sample count overflow");
+    if ((rc = capture_needs_reset(ctx))) return rc;
+    if (ctx->ptMode == RC_PT_WAVEFRONT && (rc = capture_block(ctx, "the wavefront path tracer (host-driven queues)")))
+        return rc;
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_images(ctx, cam->width, cam->height))) return rc;
     hipError_t e;
@@ -452,6 +522,7 @@ int rc_render_passes(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *p, i
 int rc_reset(rc_ctx *ctx) {
     if (!ctx) return RC_ERR_INVALID_ARG;
     ctx->totalSamples = 0;
+    if (ctx->capturing) ctx->captureResetSeen = 1;
     // Renderer.reset re-creates the textures with NULL data (zeroed storage).
     Images &im = ctx->img;
     if (im.width > 0) {
@@ -461,8 +532,8 @@ int rc_reset(rc_ctx *ctx) {
         void *ptrs[] = {im.accum, im.gbuf, im.galbedo, im.variance, im.primId, im.gbufId, im.accum64};
         for (int i = 0; i < 7; ++i) {
             if (!ptrs[i]) continue;
-            hipError_t e = hipMemsetAsync(ptrs[i], 0, sizes[i], ctx->stream);
-            if (e != hipSuccess) return hip_fail(ctx, e, "hipMemsetAsync(reset)");
+            hipError_t e = zero_fill(ctx->stream, ptrs[i], sizes[i]);
+            if (e != hipSuccess) return hip_fail(ctx, e, "reset fill");
         }
     }
     return RC_OK;
@@ -470,6 +541,7 @@ int rc_reset(rc_ctx *ctx) {
 
 int rc_debug_counters(rc_ctx *ctx, int32_t enable, uint64_t *out, int32_t n) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc = capture_block(ctx, "rc_debug_counters")) return rc;
     if (n < 0 || n > 64 || (n > 0 && !out)) return set_error(ctx, RC_ERR_INVALID_ARG, "bad counter buffer");
     (void)hipSetDevice(ctx->device);
     hipError_t e = hipSuccess;
@@ -519,6 +591,7 @@ int rc_set_pt_mode(rc_ctx *ctx, int32_t mode) {
 
 int rc_set_accum_mode(rc_ctx *ctx, int32_t mode) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc = capture_block(ctx, "rc_set_accum_mode")) return rc;
     if (mode != RC_ACCUM_FP32 && mode != RC_ACCUM_FP64)
         return set_error(ctx, RC_ERR_INVALID_ARG, "accumulation mode must be RC_ACCUM_FP32 or RC_ACCUM_FP64");
     ctx->accumMode = mode;
@@ -610,6 +683,7 @@ int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t) {
     if ((rc = validate_post(ctx, p, t))) return rc;
     if (ctx->img.width <= 0 || ctx->totalSamples <= 0)
         return set_error(ctx, RC_ERR_STATE, "nothing accumulated: render before rc_post");
+    if ((rc = capture_needs_reset(ctx))) return rc;
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_post(ctx, ctx->img.width, ctx->img.height))) return rc;
     return run_post_chain(ctx, ctx->img.accum, ctx->img.width, ctx->img.height, *p, *t, ctx->totalSamples, true);
@@ -622,6 +696,7 @@ int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height,
     if ((rc = validate_post(ctx, p, t))) return rc;
     if (!rgba || width <= 0 || height <= 0 || width > 16384 || height > 16384)
         return set_error(ctx, RC_ERR_INVALID_ARG, "image pointer/size invalid");
+    if ((rc = capture_block(ctx, "rc_post_image"))) return rc;
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_post(ctx, width, height))) return rc;
     hipError_t e = hipMemcpyAsync(ctx->img.postIn, rgba, (size_t)width * (size_t)height * 16, hipMemcpyHostToDevice,
@@ -640,6 +715,7 @@ int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height,
 
 int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc = capture_block(ctx, "rc_read")) return rc;
     if (!dst) return set_error(ctx, RC_ERR_INVALID_ARG, "dst is NULL");
     size_t need = rc_buffer_bytes(ctx, which);
     if (need == 0) return set_error(ctx, RC_ERR_STATE, "buffer not available (bad id, no render yet, or FP64 off)");
@@ -660,6 +736,107 @@ int rc_read(rc_ctx *ctx, int32_t which, void *dst, size_t bytes) {
     hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rc_read");
+    return RC_OK;
+}
+
+// ---- captured frames ----------------------------------------------------
+
+static void release_frame(rc_ctx *ctx) {
+    if (ctx->frameExec || ctx->frameGraph) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->frameExec) (void)hipGraphExecDestroy(ctx->frameExec);
+    if (ctx->frameGraph) (void)hipGraphDestroy(ctx->frameGraph);
+    ctx->frameExec = nullptr;
+    ctx->frameGraph = nullptr;
+    ctx->graphUsesViews = 0;
+}
+
+int rc_frame_capture_begin(rc_ctx *ctx) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (ctx->capturing) return set_error(ctx, RC_ERR_STATE, "a frame is already being captured");
+    (void)hipSetDevice(ctx->device);
+    release_frame(ctx);
+    ctx->nTemporal = 0;
+    ctx->captureResetSeen = 0;
+    ctx->captureViewsResetSeen = 0;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
+    ctx->capturing = 1;
+    return RC_OK;
+}
+
+int rc_frame_capture_end(rc_ctx *ctx) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (!ctx->capturing) return set_error(ctx, RC_ERR_STATE, "no frame is being captured");
+    (void)hipSetDevice(ctx->device);
+    ctx->capturing = 0;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        ctx->graphUsesViews = 0;
+        return hip_fail(ctx, e != hipSuccess ? e : hipErrorUnknown, "hipStreamEndCapture");
+    }
+    e = hipGraphInstantiate(&ctx->frameExec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        ctx->frameExec = nullptr;
+        (void)hipGraphDestroy(g);
+        ctx->graphUsesViews = 0;
+        return hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    ctx->frameGraph = g;
+    ctx->graphFreeGen = ctx->freeGen;
+    ctx->captured.totalSamples = ctx->totalSamples;
+    ctx->captured.radius = ctx->sppm.radius;
+    ctx->captured.radius64 = ctx->sppm.radius64;
+    ctx->captured.iteration = ctx->sppm.iteration;
+    ctx->captured.photonCount = ctx->sppm.photonCount;
+    ctx->captured.hashSize = ctx->sppm.hashSize;
+    ctx->captured.viewTotals = views_totals(ctx);
+    return RC_OK;
+}
+
+int rc_frame_replay(rc_ctx *ctx, const rc_temporal *t) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (ctx->capturing) return set_error(ctx, RC_ERR_STATE, "rc_frame_replay while capturing");
+    if (!ctx->frameExec) return set_error(ctx, RC_ERR_STATE, "no captured frame");
+    if (ctx->graphFreeGen != ctx->freeGen)
+        return set_error(ctx, RC_ERR_STATE,
+                         "buffers the captured frame uses were reallocated since (new resolution, scene upload, larger "
+                         "workspace): capture it again");
+    if (ctx->graphUsesViews && ctx->graphViewTablesSig != ctx->viewTablesSig)
+        return set_error(ctx, RC_ERR_STATE,
+                         "the view tables changed since the frame was captured (other views were rendered): capture "
+                         "it again");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipSuccess;
+    if (ctx->nTemporal > 0) {
+        if (t) {
+            hipLaunchKernelGGL(fill_temporal_kernel, dim3((ctx->nTemporal + 255) / 256), dim3(256), 0, ctx->stream,
+                               ctx->dTemporal, ctx->nTemporal, *t);
+            e = hipGetLastError();
+        } else {
+            e = hipMemcpyAsync(ctx->dTemporal, ctx->hTemporal, sizeof(rc_temporal) * ctx->nTemporal,
+                               hipMemcpyHostToDevice, ctx->stream);
+        }
+        if (e != hipSuccess) return hip_fail(ctx, e, "temporal inputs of the captured frame");
+    }
+    e = hipGraphLaunch(ctx->frameExec, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    ctx->totalSamples = ctx->captured.totalSamples;
+    ctx->sppm.radius = ctx->captured.radius;
+    ctx->sppm.radius64 = ctx->captured.radius64;
+    ctx->sppm.iteration = ctx->captured.iteration;
+    ctx->sppm.photonCount = ctx->captured.photonCount;
+    ctx->sppm.hashSize = ctx->captured.hashSize;
+    views_set_totals(ctx, ctx->captured.viewTotals);
+    return RC_OK;
+}
+
+int rc_frame_release(rc_ctx *ctx) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (ctx->capturing) return set_error(ctx, RC_ERR_STATE, "rc_frame_release while capturing");
+    (void)hipSetDevice(ctx->device);
+    release_frame(ctx);
     return RC_OK;
 }
 

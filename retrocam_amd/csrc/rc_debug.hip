@@ -36,6 +36,7 @@ extern "C" int rc_debug_rays(rc_ctx *ctx, const float *rays, int32_t n, int32_t 
                              int32_t *unoccludedOut) {
     using namespace rc;
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_debug_rays")) return rc_;
     if (!rays || !triOut || !tOut || !unoccludedOut || n <= 0 || n > (1 << 22))
         return set_error(ctx, RC_ERR_INVALID_ARG, "rc_debug_rays: bad arguments");
     if (!ctx->scene.valid) return set_error(ctx, RC_ERR_STATE, "no scene uploaded");
@@ -68,6 +69,7 @@ extern "C" int rc_debug_rays(rc_ctx *ctx, const float *rays, int32_t n, int32_t 
 extern "C" int rc_debug_pcg(rc_ctx *ctx, const int32_t *xyf, int32_t n, int32_t count, uint32_t *out) {
     using namespace rc;
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_debug_pcg")) return rc_;
     if (!xyf || !out || n <= 0 || count < 0 || n > (1 << 20) || count > 1024)
         return set_error(ctx, RC_ERR_INVALID_ARG, "rc_debug_pcg: bad arguments");
     (void)hipSetDevice(ctx->device);

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "rc_device.h"
 #include "retrocam_hip.h"
@@ -142,11 +143,46 @@ struct rc_ctx {
     int32_t denoiser = RC_DENOISE_ATROUS;
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
     int countersEnabled = 0;
+    // captured frame (rc_frame_*): the graph, its instance, and the temporal
+    // inputs of its post chains, one device slot per recorded chain
+    int capturing = 0;
+    int captureResetSeen = 0;
+    hipGraph_t frameGraph = nullptr;
+    hipGraphExec_t frameExec = nullptr;
+    rc_temporal *dTemporal = nullptr;  // kMaxTemporalSlots device slots
+    rc_temporal *hTemporal = nullptr;  // pinned: the values recorded at capture
+    int nTemporal = 0;                 // slots used by the recorded chains
+    uint64_t viewTablesSig = 0;        // content signature of the view tables on the device
+    uint64_t graphViewTablesSig = 0;   // signature the recorded frame reads
+    int graphUsesViews = 0;
+    // bumped before any device buffer of the context is freed (reallocation): a
+    // recorded frame whose generation differs may reference freed memory
+    uint64_t freeGen = 0;
+    uint64_t graphFreeGen = 0;
+    int captureViewsResetSeen = 0;
+    // host-side state at the end of the recorded calls, restored by each replay
+    struct {
+        int32_t totalSamples = 0;
+        float radius = 0.0f;
+        double radius64 = 0.0;
+        int iteration = 0, photonCount = 0, hashSize = 0;
+        std::vector<int> viewTotals;
+    } captured;
 };
 
 namespace rc {
 
+// Post chains one captured frame can record (one temporal slot each).
+constexpr int kMaxTemporalSlots = 4096;
+
 int set_error(rc_ctx *ctx, int code, const std::string &msg);
+// RC_ERR_STATE if a frame is being captured: `what` would allocate, wait for
+// the device or copy to the host, none of which a recorded graph can hold.
+int capture_block(rc_ctx *ctx, const char *what);
+// Temporal input of a post chain: by value when run eagerly (null slot); while
+// capturing, the next device slot, which rc_frame_replay refills before each
+// launch.
+const rc_temporal *temporal_slot(rc_ctx *ctx, const rc_temporal &t);
 int hip_fail(rc_ctx *ctx, hipError_t e, const char *what);
 
 // Launches nSamples progressive passes starting at frame ctx->totalSamples.
@@ -172,11 +208,17 @@ int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_tempor
 DeviceScene *scene_slot(rc_ctx *ctx, int slot);
 // Batched views (rc_batch.hip).
 void free_views(rc_ctx *ctx);
+// Sample counts of the current views (captured frames keep and restore them).
+std::vector<int> views_totals(const rc_ctx *ctx);
+void views_set_totals(rc_ctx *ctx, const std::vector<int> &totals);
 
 // One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
 // with the given step, 200 = display encode into rgba8.
+// dt: when non-null, the kernel reads the temporal inputs from this device slot
+// instead of t (captured frames).
 hipError_t launch_post_pass(hipStream_t stream, int pass, const float4 *src, float4 *dst, int w, int h,
                             const rc_post_params &p, const rc_temporal &t, const ushort4 *gbuf,
-                            const ushort4 *galbedo, const float *variance, int total, int step, uchar4 *rgba8);
+                            const ushort4 *galbedo, const float *variance, int total, int step, uchar4 *rgba8,
+                            const rc_temporal *dt = nullptr);
 
 }  // namespace rc

@@ -672,10 +672,12 @@ constexpr int kDwPixPerChunk = 1024;
 hipError_t ensure_ws(rc_ctx *ctx, Net &n, int N, int H, int W, bool train) {
     const size_t need = (size_t)N * H * W;
     if (need <= n.wsCap && (n.wsTrain || !train)) return hipSuccess;
+    if (ctx->capturing) return hipErrorStreamCaptureUnsupported;  // would allocate inside a captured frame
     const size_t cap = need > n.wsCap ? need : n.wsCap;
     const bool tr = train || n.wsTrain;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return e;
+    ++ctx->freeGen;
     free_ws(n);
     auto A = [&](auto *&p, size_t cnt) {
         if (e == hipSuccess) e = nalloc(p, cnt);
@@ -1008,6 +1010,7 @@ int32_t rc_net_param_count(void) {
 
 int rc_net_init(rc_ctx *ctx, uint32_t seed) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_init")) return rc_;
     (void)hipSetDevice(ctx->device);
     Net *n = get_net(ctx);
     if (!n) return set_error(ctx, RC_ERR_OOM, "host allocation failed");
@@ -1018,6 +1021,7 @@ int rc_net_init(rc_ctx *ctx, uint32_t seed) {
 
 int rc_net_set_params(rc_ctx *ctx, const float *theta, int32_t count) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_set_params")) return rc_;
     (void)hipSetDevice(ctx->device);
     Net *n = get_net(ctx);
     if (!n) return set_error(ctx, RC_ERR_OOM, "host allocation failed");
@@ -1027,6 +1031,7 @@ int rc_net_set_params(rc_ctx *ctx, const float *theta, int32_t count) {
 
 int rc_net_get_params(rc_ctx *ctx, float *theta, int32_t count) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_get_params")) return rc_;
     if (!net_ready(ctx)) return set_error(ctx, RC_ERR_STATE, "net not initialised (rc_net_init / rc_net_set_params)");
     Net &n = *ctx->net;
     if (!theta || count != n.nParams) return set_error(ctx, RC_ERR_INVALID_ARG, "parameter count mismatch");
@@ -1038,6 +1043,7 @@ int rc_net_get_params(rc_ctx *ctx, float *theta, int32_t count) {
 
 int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32_t n, int32_t h, int32_t w) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_set_dataset")) return rc_;
     (void)hipSetDevice(ctx->device);
     Net *net = get_net(ctx);
     if (!net) return set_error(ctx, RC_ERR_OOM, "host allocation failed");
@@ -1067,6 +1073,7 @@ int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32
 
 int rc_net_train(rc_ctx *ctx, int32_t steps, int32_t batch, float lr, uint32_t seed, float *losses) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_train")) return rc_;
     if (!net_ready(ctx)) return set_error(ctx, RC_ERR_STATE, "net not initialised (rc_net_init / rc_net_set_params)");
     Net &n = *ctx->net;
     if (n.nCrops <= 0) return set_error(ctx, RC_ERR_STATE, "no dataset (rc_net_set_dataset)");
@@ -1116,6 +1123,7 @@ int rc_net_train(rc_ctx *ctx, int32_t steps, int32_t batch, float lr, uint32_t s
 int rc_net_grad(rc_ctx *ctx, const float *raw, const float *target, int32_t n, int32_t h, int32_t w, float *loss,
                 float *grad) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_grad")) return rc_;
     if (!net_ready(ctx)) return set_error(ctx, RC_ERR_STATE, "net not initialised (rc_net_init / rc_net_set_params)");
     int rc = check_shape(ctx, n, h, w);
     if (rc) return rc;
@@ -1146,6 +1154,7 @@ int rc_net_grad(rc_ctx *ctx, const float *raw, const float *target, int32_t n, i
 
 int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t w, float *out) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_net_forward")) return rc_;
     if (!net_ready(ctx)) return set_error(ctx, RC_ERR_STATE, "net not initialised (rc_net_init / rc_net_set_params)");
     int rc = check_shape(ctx, n, h, w);
     if (rc) return rc;

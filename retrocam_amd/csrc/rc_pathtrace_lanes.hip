@@ -663,7 +663,10 @@ int lanes_chunk_samples(int width, int height, int nSamples, long long capBytes)
 
 hipError_t ensure_lane_buffers(rc_ctx *ctx, size_t sampleBytes, int counters) {
     LaneBuffers &L = ctx->lanes;
+    if (ctx->capturing && (sampleBytes > L.sampleBytes || counters > L.nCounters))
+        return hipErrorStreamCaptureUnsupported;  // would allocate inside a captured frame
     if (sampleBytes > L.sampleBytes) {
+        ++ctx->freeGen;
         if (L.samples) (void)hipFree(L.samples);
         L.samples = nullptr;
         L.sampleBytes = 0;
@@ -672,6 +675,7 @@ hipError_t ensure_lane_buffers(rc_ctx *ctx, size_t sampleBytes, int counters) {
         L.sampleBytes = sampleBytes;
     }
     if (counters > L.nCounters) {
+        ++ctx->freeGen;
         if (L.counters) (void)hipFree(L.counters);
         L.counters = nullptr;
         L.nCounters = 0;
@@ -696,6 +700,8 @@ hipError_t ensure_cont(rc_ctx *ctx, const void *kernel) {
     LaneBuffers &L = ctx->lanes;
     const size_t lanes = (size_t)resident_blocks(kernel) * kBlock;
     if (lanes <= L.contLanes) return hipSuccess;
+    if (ctx->capturing) return hipErrorStreamCaptureUnsupported;  // would allocate inside a captured frame
+    ++ctx->freeGen;
     if (L.cont) (void)hipFree(L.cont);
     L.cont = nullptr;
     L.contLanes = 0;

@@ -53,6 +53,7 @@ struct PostArgs {
     float4 *dst;
     rc_post_params p;
     rc_temporal t;
+    const rc_temporal *dt;  // non-null: read t from this device slot (captured frames)
     GBufs g;
     int step;        // a-trous step width
     uchar4 *rgba8;   // display output
@@ -548,6 +549,7 @@ enum { P_P00 = 0, P_ATROUS = 100, P_DISPLAY = 200 };
 
 template <int PASS>
 __global__ void __launch_bounds__(256) post_kernel(PostArgs A) {
+    if (A.dt) A.t = *A.dt;
     int x = blockIdx.x * 16 + (threadIdx.x & 15);
     int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= A.src.w || y >= A.src.h) return;
@@ -608,10 +610,11 @@ static post_fn kernel_for(int pass) {
 
 hipError_t launch_post_pass(hipStream_t stream, int pass, const float4 *src, float4 *dst, int w, int h,
                             const rc_post_params &p, const rc_temporal &t, const ushort4 *gbuf, const ushort4 *galbedo,
-                            const float *variance, int total, int step, uchar4 *rgba8) {
+                            const float *variance, int total, int step, uchar4 *rgba8, const rc_temporal *dt) {
     post_fn k = kernel_for(pass);
     if (!k) return hipErrorInvalidValue;
     PostArgs A;
+    A.dt = dt;
     A.src.p = src;
     A.src.w = w;
     A.src.h = h;

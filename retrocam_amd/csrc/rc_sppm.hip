@@ -234,10 +234,12 @@ void free_sppm(Sppm &s) {
 static int ensure_sppm(rc_ctx *ctx, int photons, int hashSize) {
     Sppm &s = ctx->sppm;
     if (s.capPhotons >= photons && s.capHash >= hashSize && s.counter) return RC_OK;
+    if (int rc = capture_block(ctx, "allocating the photon map")) return rc;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
     int cp = s.capPhotons > photons ? s.capPhotons : photons;
     int ch = s.capHash > hashSize ? s.capHash : hashSize;
+    ++ctx->freeGen;
     free_sppm(s);
     e = hipMalloc((void **)&s.photons, (size_t)cp * 48);
     if (e == hipSuccess) e = hipMalloc((void **)&s.next, (size_t)cp * 4);
@@ -323,6 +325,8 @@ int rc_sppm_gather(rc_ctx *ctx, const rc_camera *cam) {
         return set_error(ctx, RC_ERR_STATE, "no accumulation to gather into (render first)");
     if (cam->width != ctx->img.width || cam->height != ctx->img.height)
         return set_error(ctx, RC_ERR_INVALID_ARG, "camera resolution differs from the accumulation buffer");
+    if (ctx->capturing && !ctx->captureResetSeen)
+        return set_error(ctx, RC_ERR_STATE, "a captured frame must begin with rc_reset before gathering into it");
     (void)hipSetDevice(ctx->device);
     GatherArgs A;
     A.S = scene_view(ctx->scene);
@@ -372,6 +376,7 @@ int rc_sppm_get_state(const rc_ctx *ctx, float *radius, int32_t *iteration, doub
 
 int rc_sppm_read(rc_ctx *ctx, float *photons, int32_t *heads, int32_t *next, uint32_t *counter) {
     if (!ctx) return RC_ERR_INVALID_ARG;
+    if (int rc = capture_block(ctx, "rc_sppm_read")) return rc;
     Sppm &s = ctx->sppm;
     if (!s.counter || s.photonCount <= 0) return set_error(ctx, RC_ERR_STATE, "rc_sppm_trace has not run");
     (void)hipSetDevice(ctx->device);

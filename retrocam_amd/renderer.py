@@ -59,6 +59,25 @@ class RenderContext:
     def stream(self) -> int:
         return self._lib.rc_stream(self.h)
 
+    def capture_begin(self):
+        """Starts recording the device work of the following calls into a frame
+        graph (rc_frame_capture_begin); it must begin with a reset."""
+        self.check(self._lib.rc_frame_capture_begin(self.h))
+
+    def capture_end(self):
+        self.check(self._lib.rc_frame_capture_end(self.h))
+
+    def replay(self, temporal=None):
+        """Launches the recorded frame; temporal (TemporalInputs or RcTemporal)
+        replaces the post chains' time inputs, None keeps the recorded ones."""
+        t = None
+        if temporal is not None:
+            t = temporal.to_c() if hasattr(temporal, "to_c") else temporal
+        self.check(self._lib.rc_frame_replay(self.h, C.byref(t) if t is not None else None))
+
+    def release_frame(self):
+        self.check(self._lib.rc_frame_release(self.h))
+
     def sync(self):
         self.check(self._lib.rc_sync(self.h))
 
