@@ -173,6 +173,22 @@ def serving(ctx, net, batches, log):
         return clk.elapsed_ms()
 
     batch_once()  # warm-up: arenas and the network workspace reach their largest size
+
+    # the whole batch (one trace launch + every view's chain) captured once, replayed
+    ctx.capture_begin()
+    vb.render(vreqs, reset=True)
+    for i in range(len(vreqs)):
+        vb.post(i, pp, tmp)
+    ctx.capture_end()
+    ctx.replay()
+    ctx.sync()
+    tg0 = time.perf_counter()
+    for _ in range(batches):
+        ctx.replay()
+    ctx.sync()
+    graph_batch_ms = (time.perf_counter() - tg0) / batches * 1e3
+    ctx.release_frame()
+    log(f"[bench] serving batch as a graph replay: {graph_batch_ms:.1f} ms")
     lat_b, lat_s, tb, ts = [], [], [], []
     for b in range(batches):
         lat = batch_once()
@@ -196,6 +212,8 @@ def serving(ctx, net, batches, log):
                        "p50_ms": round(float(np.percentile(lat_s, 50)), 1),
                        "p99_ms": round(float(np.percentile(lat_s, 99)), 1), "pass_ms": round(float(np.mean(ts)), 1)},
         "speedup": round(float(np.mean(ts)) / float(np.mean(tb)), 3),
+        "graph_batch_ms": round(graph_batch_ms, 1),
+        "graph_requests_per_s": round(len(vreqs) / (graph_batch_ms / 1e3), 3),
         "chain": "one persistent trace launch for all views, then per view: p00, U-Net, 14 VHS passes, display",
     }
 
@@ -212,6 +230,8 @@ def sppm_leg(ctx, iters, log):
     w = workloads.sppm_caustics()
     SceneUploader(ctx).upload(w.scene)
     r = Renderer(ctx)
+    tuning = r.autotune(w.camera, w.params, samples=1)  # the leg renders one-sample passes
+    log(f"[bench] sppm scene tuning: {tuning}")
     s = SPPMManager(ctx, init_radius=w.extra["init_radius"], fp64_radius=True)
     P, Hs, alpha = w.extra["photons"], w.extra["hash_size"], w.extra["alpha"]
 
@@ -234,16 +254,47 @@ def sppm_leg(ctx, iters, log):
     tp, tt, tg, stored = run(iters, True)
     ctx.sync()
     wall = time.perf_counter() - t0
+
+    # The whole SPPM frame (iterations, then p00 + 14 VHS passes + display) as one
+    # captured graph, replayed.
+    from retrocam_amd.post import VHS_PASSES, PostProcessStack, TemporalInputs, make_params
+    post = PostProcessStack(ctx)
+    pp = make_params(enable=VHS_PASSES)
+
+    def frame():
+        r.reset()
+        s.reset(w.extra["init_radius"])
+        for _ in range(iters):
+            s.trace_photons(P, Hs)
+            r.render(w.camera, w.params, n_samples=1)
+            s.gather_radiance(w.camera)
+            s.update_radius(alpha)
+        post.run_on_accum(pp, TemporalInputs(), read=False)
+
+    frame()
+    ctx.capture_begin()
+    frame()
+    ctx.capture_end()
+    ctx.replay()
+    ctx.sync()
+    tg0 = time.perf_counter()
+    for _ in range(3):
+        ctx.replay()
+    ctx.sync()
+    graph_frame_s = (time.perf_counter() - tg0) / 3
+    ctx.release_frame()
     npx = w.camera.width * w.camera.height
     st = int(np.mean(stored))
     ins_bytes = st * (48 + 4 + 8) + Hs * 4  # record + next + atomicExch RMW per stored photon, heads clear
     out = {"iterations": iters, "photons_per_iteration": P, "hash_cells": Hs, "stored_photons_per_iteration": st,
-           "frame_s": round(wall, 3), "photon_trace_ms": round(float(np.mean(tp)), 3),
+           "frame_s": round(wall, 3), "graph_frame_s": round(graph_frame_s, 4),
+           "graph": "iterations + p00 + 14 VHS passes + display captured once, replayed",
+           "photon_trace_ms": round(float(np.mean(tp)), 3),
            "path_trace_ms": round(float(np.mean(tt)), 3), "gather_ms": round(float(np.mean(tg)), 3),
            "mphotons_per_s": round(P / (float(np.mean(tp)) / 1e3) / 1e6, 1),
            "gather_mpixels_per_s": round(npx / (float(np.mean(tg)) / 1e3) / 1e6, 1),
            "insert_gbs_compulsory": round(ins_bytes / (float(np.mean(tp)) / 1e3) / 1e9, 1),
-           "radius_final": s.search_radius}
+           "radius_final": s.search_radius, "tuning": tuning}
     log(f"[bench] sppm: {out}")
     return out
 
@@ -283,6 +334,8 @@ def main():
     ap.add_argument("--sppm-iters", type=int, default=64, help="SPPM caustics iterations to time (0 = skip)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 1 GiB device copies (calibrates FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc)")
+    ap.add_argument("--eager", action="store_true", help="time eager calls instead of replays of the captured frame")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default lane-scheduler tuning")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -310,6 +363,10 @@ def main():
     r = Renderer(ctx)
     pt_mode = {"lanes": N.RC_PT_LANES, "wavefront": N.RC_PT_WAVEFRONT, "megakernel": N.RC_PT_MEGAKERNEL}[args.pt_mode]
     r.set_pt_mode(pt_mode)
+    tuning = None
+    if args.pt_mode == "lanes" and not args.no_autotune:
+        tuning = r.autotune(w.camera, w.params, samples=8)
+        log(f"[bench] autotuned lane scheduler: {tuning}")
     net = NetDenoiser(ctx).load()
     post = PostProcessStack(ctx)
     pp_full = make_params(enable=VHS_PASSES, atrous=True, iterations=5)
@@ -400,11 +457,31 @@ def main():
 
     for _ in range(args.warmup):
         frame(spp, method)
+    # The frame chain (reset, spp samples, p00, denoiser, 14 VHS passes, display)
+    # recorded once as a HIP graph; a timed step is one replay.
+    if not args.eager:
+        frame(spp, method)  # buffers of this frame exist before it is captured
+        ctx.capture_begin()
+        frame(spp, method)
+        ctx.capture_end()
+        ctx.replay()
     ctx.sync()
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if args.eager:
+            frame(spp, method)
+        else:
+            ctx.replay()
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    fps = args.steps / wall
+    # the same frames eagerly, with hipEvents around the trace and the post chain
     t_trace = HipTimer(ctx.stream)
     t_post = HipTimer(ctx.stream)
-    t0 = time.perf_counter()
+    te = time.perf_counter()
     for _ in range(args.steps):
         t_trace.start()
         trace(spp)
@@ -414,12 +491,9 @@ def main():
         post.run_on_accum(make_params(enable=VHS_PASSES, atrous=True, iterations=5), temporal, read=False)
         t_post.stop()
     ctx.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    eager_fps = args.steps / (time.perf_counter() - te)
     kms = t_trace.times_ms()
     pms = t_post.times_ms()
-    ms_per_step = wall / args.steps * 1e3
-    fps = args.steps / wall
     avg_kernel_s = float(np.mean(kms)) / 1e3
     achieved = alg_bytes / avg_kernel_s / 1e9
     post_ms = float(np.mean(pms))
@@ -438,6 +512,8 @@ def main():
         log("[bench] copy calibration: 3 x 1 GiB device-to-device copies done")
 
     sppm = sppm_leg(ctx, args.sppm_iters, log) if args.sppm_iters > 0 else None
+    if tuning:  # the SPPM leg tuned for its own scene; serving uses the full frame's tuning
+        r.set_tuning(**{k: tuning[k] for k in Renderer.TUNING_FIELDS})
 
     serve = None
     if args.serving > 0:
@@ -471,6 +547,9 @@ def main():
                    "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
                    "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU",
                    "pt_mode": args.pt_mode},
+        "frame": "replay of the captured frame graph" if not args.eager else "eager calls",
+        "eager_frames_per_s": round(eager_fps, 4),
+        "tuning": tuning,
         "mrays_per_s": round(rays * 1e3 / float(np.mean(kms)) / 1e6, 2),
         "rays_per_frame": rays,
         "trace_ms": round(float(np.mean(kms)), 3),

@@ -369,19 +369,31 @@ int rc_frame_release(rc_ctx *ctx);
 
 /* ---- tuning (no counterpart in the reference) --------------------------- */
 /* Launch parameters that change speed, never results. */
-#define RC_LANE_VARIANTS 6
+#define RC_LANE_VARIANTS 9
 typedef struct rc_tuning {
     int32_t laneThreshold;  /* RC_PT_LANES: a wave stops walking and shades once this
-                               many of its 64 lanes wait (1..64, default 40)       */
+                               many of its 64 lanes wait (1..64, default 32)       */
     int32_t laneVariant;    /* RC_PT_LANES kernel variant, 0..RC_LANE_VARIANTS-1:
                                occupancy, LDS stack depth, triangles per step
-                               (default 4: 7 waves/SIMD, 11 entries, one triangle) */
+                               (default 7: 4 waves/SIMD, 20 entries, one triangle) */
     int32_t laneBufferMiB;  /* RC_PT_LANES sample buffer bound (16 B per pixel and
                                sample; larger launches run in chunks), 64..65536,
                                default 4096                                        */
+    int32_t laneTileWidth;  /* RC_PT_LANES unit tile: laneTileWidth x (64 /
+                               laneTileWidth) pixels; 4, 8 (default), 16 or 32     */
+    int32_t ptMode;         /* path-trace form (rc_set_pt_mode); rc_set_tuning
+                               sets it too                                         */
 } rc_tuning;
 int rc_get_tuning(const rc_ctx *ctx, rc_tuning *out);
 int rc_set_tuning(rc_ctx *ctx, const rc_tuning *t);
+/* Autotuner: times the lane scheduler's variants, wait thresholds and tile
+ * shapes, and the megakernel, on the uploaded scene at cam's resolution
+ * (samplesPerTrial samples per trial, 1..4096; every candidate renders the same
+ * image) and keeps the fastest as the context's tuning and path-trace form. out and msPerSample (may be NULL) receive it and its
+ * time per sample. The accumulation is reset on return. RC_ERR_STATE for a scene
+ * the lane scheduler does not run (BVH of 30 levels or more). */
+int rc_autotune(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *p, int32_t samplesPerTrial,
+                rc_tuning *out, float *msPerSample);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------ */
 /* Work counters of the path-trace kernel, for the roofline byte model. Copies

@@ -39,7 +39,7 @@ RC_ACCUM_FP64 = 1
 RC_PT_MEGAKERNEL = 0
 RC_PT_WAVEFRONT = 1
 RC_PT_LANES = 2
-RC_LANE_VARIANTS = 6
+RC_LANE_VARIANTS = 9
 
 RC_DENOISE_ATROUS = 0
 RC_DENOISE_NET = 1
@@ -96,7 +96,8 @@ RC_MAX_SCENE_SLOTS = 64
 
 
 class RcTuning(C.Structure):
-    _fields_ = [("laneThreshold", C.c_int32), ("laneVariant", C.c_int32), ("laneBufferMiB", C.c_int32)]
+    _fields_ = [("laneThreshold", C.c_int32), ("laneVariant", C.c_int32), ("laneBufferMiB", C.c_int32),
+                ("laneTileWidth", C.c_int32), ("ptMode", C.c_int32)]
 
 
 class RcTemporal(C.Structure):
@@ -163,6 +164,8 @@ HIP_SYMBOLS = {
     "rc_frame_release": (C.c_int, [_VP]),
     "rc_get_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_set_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
+    "rc_autotune": (C.c_int, [_VP, C.POINTER(RcCamera), C.POINTER(RcPtParams), C.c_int32, C.POINTER(RcTuning),
+                              C.POINTER(C.c_float)]),
     "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),
     "rc_debug_pcg": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
     "rc_debug_rays": (C.c_int, [_VP, _FP, C.c_int32, C.POINTER(C.c_int32), _FP, C.POINTER(C.c_int32)]),

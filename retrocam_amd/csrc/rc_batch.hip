@@ -181,7 +181,7 @@ int plan_lanes(rc_ctx *ctx, ViewTables &T) {
     int maxS = 0;
     for (int k = 0; k < n; ++k) {
         const CamArgs &c = T.args[k].cam;
-        tilePx[k] = (long long)((c.width + 7) / 8) * ((c.height + 7) / 8) * 64;
+        tilePx[k] = LaneTile(c.width, c.height, T.args[k].tileShift).pixels();
         maxS = std::max(maxS, T.args[k].nSamples);
     }
     auto bytesFor = [&](long long K) {
@@ -373,6 +373,7 @@ int rc_views_render(rc_ctx *ctx, const rc_view *views, int32_t n, int32_t flags)
         A.primId = V->primId + o;
         A.gbufId = V->gbufId + o;
         A.counters = ctx->counters;
+        A.tileShift = ctx->laneTileShift;
         T.tileOff[k] = (int)tot;
         tot += tiles(i);
     }

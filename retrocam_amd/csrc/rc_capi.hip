@@ -563,6 +563,8 @@ int rc_get_tuning(const rc_ctx *ctx, rc_tuning *out) {
     out->laneThreshold = ctx->laneThresh;
     out->laneVariant = ctx->laneVariant;
     out->laneBufferMiB = ctx->laneBufferMiB;
+    out->laneTileWidth = 1 << ctx->laneTileShift;
+    out->ptMode = ctx->ptMode;
     return RC_OK;
 }
 
@@ -575,6 +577,14 @@ int rc_set_tuning(rc_ctx *ctx, const rc_tuning *t) {
         return set_error(ctx, RC_ERR_INVALID_ARG, "laneVariant out of range");
     if (t->laneBufferMiB < 64 || t->laneBufferMiB > 65536)
         return set_error(ctx, RC_ERR_INVALID_ARG, "laneBufferMiB must be in 64..65536");
+    int shift = -1;
+    for (int k = 2; k <= 5; ++k)
+        if (t->laneTileWidth == (1 << k)) shift = k;
+    if (shift < 0) return set_error(ctx, RC_ERR_INVALID_ARG, "laneTileWidth must be 4, 8, 16 or 32");
+    if (t->ptMode != RC_PT_MEGAKERNEL && t->ptMode != RC_PT_WAVEFRONT && t->ptMode != RC_PT_LANES)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "ptMode must be RC_PT_LANES, RC_PT_MEGAKERNEL or RC_PT_WAVEFRONT");
+    ctx->laneTileShift = shift;
+    ctx->ptMode = t->ptMode;
     ctx->laneThresh = t->laneThreshold;
     ctx->laneVariant = t->laneVariant;
     ctx->laneBufferMiB = t->laneBufferMiB;

@@ -134,11 +134,12 @@ struct rc_ctx {
     rc::Wavefront wf;
     int32_t ptMode = RC_PT_LANES;
     rc::LaneBuffers lanes;
-    int32_t laneThresh = 40;  // lanes waiting before a wave shades (rc_pathtrace_lanes.hip)
-    int32_t laneVariant = 4;  // lane-kernel variant (rc_pathtrace_lanes.hip)
+    int32_t laneThresh = 32;  // lanes waiting before a wave shades (rc_pathtrace_lanes.hip)
+    int32_t laneVariant = 7;  // lane-kernel variant (rc_pathtrace_lanes.hip)
     // Upper bound of the lane scheduler's sample buffer (16 B per pixel and
     // sample): a launch with more samples x pixels runs in chunks.
     int32_t laneBufferMiB = 4096;
+    int32_t laneTileShift = 3;  // unit tiles of 8 x 8 pixels
     rc::Net *net = nullptr;
     int32_t denoiser = RC_DENOISE_ATROUS;
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)

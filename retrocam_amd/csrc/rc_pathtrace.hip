@@ -249,6 +249,7 @@ PassArgs pass_args(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int
     A.primId = ctx->img.primId;
     A.gbufId = ctx->img.gbufId;
     A.counters = ctx->counters;
+    A.tileShift = ctx->laneTileShift;
     return A;
 }
 

@@ -236,8 +236,8 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
     const CamArgs &cam = A.cam;
     const int lane = threadIdx.x & 63;
     const bool lca = (fabsf(cam.lca[0]) + fabsf(cam.lca[2])) > 1e-4f;
-    const int tilesX = (cam.width + 7) >> 3, tilesY = (cam.height + 7) >> 3;
-    const int nTilePx = tilesX * tilesY * 64;
+    const LaneTile tl(cam.width, cam.height, A.tileShift);
+    const int nTilePx = tl.pixels();
     const int nUnits = nTilePx * K;  // < 2^30, checked by the host
     const unsigned long long below = (1ull << lane) - 1ull;
     const size_t nLanes = (size_t)gridDim.x * kBlock;
@@ -276,8 +276,8 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                     const int ts = u >> 6;  // tile * K + sample
                     const int tile = ts / K;
                     s = ts - tile * K;
-                    const int px = (tile % tilesX) * 8 + (within & 7);
-                    const int py = (tile / tilesX) * 8 + (within >> 3);
+                    int px, py;
+                    tl.pixel(tile, within, px, py);
                     pxy = (uint32_t)px | ((uint32_t)py << 16);
                     if (px < cam.width && py < cam.height) phase = PH_START;
                 }
@@ -386,8 +386,7 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
             if (endPath) {
                 // channel radiance: the sample itself, or one LCA channel whose partial
                 // results wait in the sample's slot (pathtrace.comp:611-637)
-                const size_t slot = (size_t)s * (size_t)nTilePx +
-                                    (size_t)((((py >> 3) * tilesX + (px >> 3)) << 6) | ((py & 7) << 3) | (px & 7));
+                const size_t slot = (size_t)s * (size_t)nTilePx + (size_t)tl.slot(px, py);
                 if (lca && ch < 2) {
                     float4 v = make_float4(rad.x, 0.0f, 0.0f, 0.0f);
                     if (ch == 1) {
@@ -463,8 +462,11 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
 // NS (W workgroups x NS x 2 KB of stacks within a CU's 160 KB) and whether a
 // walk step tests a whole leaf or one triangle (TRI):
 //   0: W 7, NS 11            1: W 6, NS 13  2: W 8, NS 10  3: W 5, NS 16
-//   4: W 7, NS 11, TRI (default; 1.34x variant 0 on the 250k-triangle frame)
-//   5: W 6, NS 13, TRI
+//   4: W 7, NS 11, TRI       5: W 6, NS 13, TRI       6: W 5, NS 16, TRI
+//   7: W 4, NS 20, TRI (default)                      8: W 3, NS 26, TRI
+// On the 250k-triangle frame at 80 samples: variant 0 177 ms, 4 141 ms, 6 138 ms,
+// 7 121 ms (126 VGPRs, no register spills; the 5-wave and higher variants spill
+// around the shading block and move 5x the HBM bytes in scratch traffic), 8 144 ms.
 template <bool STATS, int W, int NS, bool TRI>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
 pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh, float4 *cont) {
@@ -508,6 +510,9 @@ using LaneViewsKernel = void (*)(const PassArgs *, const LaneChunk *, int, int, 
         case 3: return S ? K<true, 5, 16, false> : K<false, 5, 16, false>;       \
         case 4: return S ? K<true, 7, 11, true> : K<false, 7, 11, true>;         \
         case 5: return S ? K<true, 6, 13, true> : K<false, 6, 13, true>;         \
+        case 6: return S ? K<true, 5, 16, true> : K<false, 5, 16, true>;         \
+        case 7: return S ? K<true, 4, 20, true> : K<false, 4, 20, true>;         \
+        case 8: return S ? K<true, 3, 26, true> : K<false, 3, 26, true>;         \
         default: return S ? K<true, 7, 11, false> : K<false, 7, 11, false>;      \
     }
 
@@ -524,8 +529,7 @@ __global__ void __launch_bounds__(256) lanes_reduce_kernel(ReduceArgs R) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= R.width * R.height) return;
     const int px = p % R.width, py = p / R.width;
-    const int tilesX = (R.width + 7) >> 3;
-    const size_t q = (size_t)(((py >> 3) * tilesX + (px >> 3)) * 64 + (py & 7) * 8 + (px & 7));
+    const size_t q = (size_t)LaneTile(R.width, R.height, R.tileShift).slot(px, py);
     float4 acc = make_float4(0, 0, 0, 0);
     double4 acc64 = make_double4(0, 0, 0, 0);
     float var = 0.0f;
@@ -579,8 +583,7 @@ __global__ void __launch_bounds__(256) lanes_reduce_views_kernel(const PassArgs 
     const int p = (int)(g - pixOff[lo]);
     const int W = A.cam.width;
     const int px = p % W, py = p / W;
-    const int tilesX = (W + 7) >> 3;
-    const size_t q = (size_t)(((py >> 3) * tilesX + (px >> 3)) * 64 + (py & 7) * 8 + (px & 7));
+    const size_t q = (size_t)LaneTile(W, A.cam.height, A.tileShift).slot(px, py);
     const int frame0 = A.frame0 + ck.s0;
     float4 acc = make_float4(0, 0, 0, 0);
     double4 acc64 = make_double4(0, 0, 0, 0);
@@ -652,8 +655,8 @@ int persistent_blocks(const void *kernel, int units) {
 
 bool lanes_supported(const DeviceScene &D) { return D.valid && D.maxDepth < 30; }
 
-int lanes_chunk_samples(int width, int height, int nSamples, long long capBytes) {
-    const long long tilePx = (long long)((width + 7) / 8) * ((height + 7) / 8) * 64;
+int lanes_chunk_samples(int width, int height, int tileShift, int nSamples, long long capBytes) {
+    const long long tilePx = LaneTile(width, height, tileShift).pixels();
     long long k = capBytes / (tilePx * 16);
     const long long kUnits = (1ll << 30) / tilePx;
     if (k > kUnits) k = kUnits;
@@ -714,8 +717,8 @@ hipError_t ensure_cont(rc_ctx *ctx, const void *kernel) {
 
 hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A0, int thresh, int variant) {
     const int W = A0.cam.width, H = A0.cam.height;
-    const int nTilePx = ((W + 7) / 8) * ((H + 7) / 8) * 64;
-    const int kMax = lanes_chunk_samples(W, H, A0.nSamples, (long long)ctx->laneBufferMiB << 20);
+    const int nTilePx = LaneTile(W, H, A0.tileShift).pixels();
+    const int kMax = lanes_chunk_samples(W, H, A0.tileShift, A0.nSamples, (long long)ctx->laneBufferMiB << 20);
     hipError_t e = ensure_lane_buffers(ctx, (size_t)kMax * (size_t)nTilePx * 16, 1);
     if (e != hipSuccess) return e;
     const bool stats = ctx->countersEnabled && ctx->counters;
@@ -738,6 +741,7 @@ hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A0, int thresh, int variant
         R.frame0 = A.frame0;
         R.width = W;
         R.height = H;
+        R.tileShift = A.tileShift;
         R.accum = A.accum;
         R.accum64 = A.accum64;
         R.variance = A.variance;

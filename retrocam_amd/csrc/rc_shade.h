@@ -34,6 +34,25 @@ struct PassArgs {
     int *primId;
     int *gbufId;
     unsigned long long *counters;
+    int tileShift;  // lane scheduler: unit tiles of (1 << tileShift) x (64 >> tileShift) pixels
+};
+
+// Unit tiles of the lane scheduler: 64 pixels each, tw = 1 << shift wide and
+// th = 64 >> shift high; a view's units are numbered tile by tile.
+struct LaneTile {
+    int shift, tilesX, tilesY;
+    __host__ __device__ LaneTile(int W, int H, int s)
+        : shift(s), tilesX((W + (1 << s) - 1) >> s), tilesY((H + (64 >> s) - 1) >> (6 - s)) {}
+    __host__ __device__ int pixels() const { return tilesX * tilesY * 64; }
+    // position of pixel (px, py) in the view's tile order
+    __host__ __device__ int slot(int px, int py) const {
+        return ((((py >> (6 - shift)) * tilesX + (px >> shift)) << 6) | ((py & ((64 >> shift) - 1)) << shift) |
+                (px & ((1 << shift) - 1)));
+    }
+    __host__ __device__ void pixel(int tile, int within, int &px, int &py) const {
+        px = (tile % tilesX) * (1 << shift) + (within & ((1 << shift) - 1));
+        py = (tile / tilesX) * (64 >> shift) + (within >> shift);
+    }
 };
 
 // Lane-scheduler form (rc_pathtrace_lanes.hip): one chunk of a view's samples.
@@ -50,7 +69,7 @@ struct LaneChunk {
 // Per-pixel frame-order accumulation of one chunk (lanes_reduce_kernel).
 struct ReduceArgs {
     const float4 *samples;
-    int nTilePx, nSamples, frame0, width, height;
+    int nTilePx, nSamples, frame0, width, height, tileShift;
     float4 *accum;
     double4 *accum64;
     float *variance;
@@ -299,7 +318,7 @@ hipError_t launch_pathtrace_views(rc_ctx *ctx, const PassArgs *views, const int 
 // lanes_chunk_samples (sample buffer <= rc_ctx::laneBufferMiB); launch_lanes_views
 // runs one chunk of many views.
 bool lanes_supported(const DeviceScene &D);
-int lanes_chunk_samples(int width, int height, int nSamples, long long capBytes);
+int lanes_chunk_samples(int width, int height, int tileShift, int nSamples, long long capBytes);
 hipError_t ensure_lane_buffers(rc_ctx *ctx, size_t sampleBytes, int counters);
 void free_lane_buffers(LaneBuffers &L);
 hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A, int thresh, int variant);

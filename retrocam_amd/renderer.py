@@ -132,12 +132,27 @@ class Renderer:
     def set_pt_mode(self, mode: int):
         self.ctx.check(self._lib.rc_set_pt_mode(self.ctx.h, int(mode)))
 
-    TUNING_FIELDS = ("laneThreshold", "laneVariant", "laneBufferMiB")
+    TUNING_FIELDS = ("laneThreshold", "laneVariant", "laneBufferMiB", "laneTileWidth", "ptMode")
 
     def get_tuning(self) -> dict:
         t = N.RcTuning()
         self.ctx.check(self._lib.rc_get_tuning(self.ctx.h, C.byref(t)))
         return {k: int(getattr(t, k)) for k in self.TUNING_FIELDS}
+
+    def autotune(self, camera: CameraSpec, params: PtParams, samples: int = 8) -> dict:
+        """rc_autotune: keeps the fastest lane-scheduler tuning for this scene and
+        resolution; returns it with its milliseconds per sample. Resets the
+        accumulation."""
+        cam = camera.to_c()
+        p = params.to_c()
+        t = N.RcTuning()
+        ms = C.c_float()
+        self.ctx.check(self._lib.rc_autotune(self.ctx.h, C.byref(cam), C.byref(p), int(samples), C.byref(t),
+                                             C.byref(ms)))
+        self.width, self.height = camera.width, camera.height
+        out = {k: int(getattr(t, k)) for k in self.TUNING_FIELDS}
+        out["ms_per_sample"] = float(ms.value)
+        return out
 
     def set_tuning(self, **kw):
         """Sets any of laneThreshold, laneVariant, laneBufferMiB (the rest stay)."""

@@ -176,12 +176,35 @@ def test_lane_tuning_changes_no_result(gpu_ctx, cornell):
         got = gpu_render(gpu_ctx, w.scene, cam, w.params, 4, batch=4, mode=N.RC_PT_LANES)
         for k in ref:
             assert np.array_equal(ref[k], got[k]), (v, k)
-    r.set_tuning(laneThreshold=40, laneVariant=4, laneBufferMiB=4096)
+    for tw in (4, 16, 32):
+        r.set_tuning(laneVariant=7, laneTileWidth=tw)
+        got = gpu_render(gpu_ctx, w.scene, cam, w.params, 4, batch=4, mode=N.RC_PT_LANES)
+        for k in ref:
+            assert np.array_equal(ref[k], got[k]), (tw, k)
+    r.set_tuning(laneThreshold=32, laneVariant=7, laneBufferMiB=4096, laneTileWidth=8, ptMode=N.RC_PT_LANES)
     for bad in ({"laneThreshold": 0}, {"laneThreshold": 65}, {"laneVariant": N.RC_LANE_VARIANTS},
-                {"laneBufferMiB": 63}):
+                {"laneBufferMiB": 63}, {"laneTileWidth": 2}, {"laneTileWidth": 12}):
         with pytest.raises(RenderError) as e:
             r.set_tuning(**bad)
         assert e.value.code == N.RC_ERR_INVALID_ARG
+
+
+def test_autotune_keeps_a_valid_tuning(gpu_ctx, cornell):
+    w = cornell
+    cam = CameraSpec(**{**w.camera.__dict__, "width": 160, "height": 90})
+    ref = gpu_render(gpu_ctx, w.scene, cam, w.params, 3, batch=3, mode=N.RC_PT_MEGAKERNEL)
+    r = Renderer(gpu_ctx)
+    r.set_pt_mode(N.RC_PT_LANES)
+    got = r.autotune(cam, w.params, samples=2)
+    assert 0 <= got["laneVariant"] < N.RC_LANE_VARIANTS and 1 <= got["laneThreshold"] <= 64
+    assert got["laneTileWidth"] in (4, 8, 16, 32) and got["ms_per_sample"] > 0.0
+    assert got["ptMode"] in (N.RC_PT_LANES, N.RC_PT_MEGAKERNEL)
+    assert r.get_tuning() == {k: got[k] for k in Renderer.TUNING_FIELDS}
+    assert r.get_total_samples() == 0
+    out = gpu_render(gpu_ctx, w.scene, cam, w.params, 3, batch=3, mode=N.RC_PT_LANES)
+    for k in ref:
+        assert np.array_equal(ref[k], out[k]), k
+    r.set_tuning(laneThreshold=32, laneVariant=7, laneTileWidth=8, ptMode=N.RC_PT_LANES)
 
 
 def test_lanes_deep_bvh_falls_back(gpu_ctx):
