@@ -299,11 +299,15 @@ def sppm_leg(ctx, iters, log):
     return out
 
 
-def read_traffic():
+def read_traffic(pt_mode):
     """HBM bytes per sample of a full-frame path-trace launch, from the committed
-    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of
-    known size, divided by the launch's samples)."""
-    path = os.path.join(ROOT, "profiles", "pathtrace_kernel__full_frame__pmc.md")
+    rocprofv3 PMC summary of that kernel (FETCH_SIZE + WRITE_SIZE scaled by a copy
+    kernel of known size, divided by the launch's samples)."""
+    name = {"lanes": "pathtrace_lanes_kernel__full_frame__pmc.md",
+            "megakernel": "pathtrace_kernel__full_frame__pmc.md"}.get(pt_mode)
+    if name is None:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_spp:"):
@@ -525,7 +529,10 @@ def main():
         log(f"[bench] CPU baseline: oracle, {args.cpu_spp} spp ...")
         cpu = cpu_baseline(w, args.cpu_spp, spp)
 
-    traffic_spp = read_traffic()
+    # the form that ran (the autotuner may choose the megakernel)
+    ran = {N.RC_PT_LANES: "lanes", N.RC_PT_MEGAKERNEL: "megakernel", N.RC_PT_WAVEFRONT: "wavefront"}[
+        tuning["ptMode"] if tuning else pt_mode]
+    traffic_spp = read_traffic(ran)
     out = {
         "metric": "854x480 frames/s at relMSE<=0.01 vs 4096spp (net vs a-trous); Mrays/s",
         "value": round(fps, 4),
@@ -546,7 +553,7 @@ def main():
                    "spp": spp, "denoiser": method, "quality_met": quality_met,
                    "max_bounces": w.params.maxBounces, "triangles": w.scene.nTris,
                    "lights": w.scene.nLights, "lens": "35mm f/2.8, 6 blades, SA 0.04", "parallelism": "single GPU",
-                   "pt_mode": args.pt_mode},
+                   "pt_mode": ran},
         "frame": "replay of the captured frame graph" if not args.eager else "eager calls",
         "eager_frames_per_s": round(eager_fps, 4),
         "tuning": tuning,
@@ -563,7 +570,7 @@ def main():
                      "traffic": int(traffic_spp * spp) if traffic_spp else None,
                      "kernel": {"lanes": "pathtrace_lanes_kernel (+ lanes_reduce_kernel)",
                                 "megakernel": "pathtrace_kernel",
-                                "wavefront": "wavefront frame"}[args.pt_mode],
+                                "wavefront": "wavefront frame"}[ran],
                      "alg_bytes_per_launch": int(alg_bytes),
                      "avg_launch_ms": round(avg_kernel_s * 1e3, 3)},
         "cpu_baseline": cpu,

@@ -189,6 +189,19 @@ def test_lane_tuning_changes_no_result(gpu_ctx, cornell):
         assert e.value.code == N.RC_ERR_INVALID_ARG
 
 
+def test_lanes_chunked_launch(gpu_ctx, cornell):
+    # 320x180 needs 0.74 MB of sample buffer per sample: with a 64 MiB bound a
+    # 100-sample launch runs as two chunks, each reduced in frame order.
+    w = cornell
+    ref = gpu_render(gpu_ctx, w.scene, w.camera, w.params, 100, batch=100, mode=N.RC_PT_MEGAKERNEL)
+    r = Renderer(gpu_ctx)
+    r.set_tuning(laneBufferMiB=64)
+    got = gpu_render(gpu_ctx, w.scene, w.camera, w.params, 100, batch=100, mode=N.RC_PT_LANES)
+    r.set_tuning(laneBufferMiB=4096)
+    for k in ref:
+        assert np.array_equal(ref[k], got[k]), k
+
+
 def test_autotune_keeps_a_valid_tuning(gpu_ctx, cornell):
     w = cornell
     cam = CameraSpec(**{**w.camera.__dict__, "width": 160, "height": 90})
