@@ -330,7 +330,7 @@ def main():
                     help="time frames at this spp (with --denoiser) instead of the U-Net's operating point")
     ap.add_argument("--denoiser", choices=["atrous", "net"], default="atrous", help="denoiser of --fixed-spp frames")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
-    ap.add_argument("--sweep", default="4,8,16,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
+    ap.add_argument("--sweep", default="4,6,8,10,12,14,16,20,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
                     help="spp values searched for the relMSE <= 0.01 operating points")
     ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["lanes", "wavefront", "megakernel"], default="lanes")
@@ -338,6 +338,7 @@ def main():
     ap.add_argument("--sppm-iters", type=int, default=64, help="SPPM caustics iterations to time (0 = skip)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 1 GiB device copies (calibrates FETCH_SIZE/WRITE_SIZE under rocprofv3 --pmc)")
+    ap.add_argument("--weights", default=None, help="denoiser weights (.npy; default: the committed ones)")
     ap.add_argument("--eager", action="store_true", help="time eager calls instead of replays of the captured frame")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default lane-scheduler tuning")
     ap.add_argument("--quiet", action="store_true")
@@ -371,7 +372,7 @@ def main():
     if args.pt_mode == "lanes" and not args.no_autotune:
         tuning = r.autotune(w.camera, w.params, samples=8)
         log(f"[bench] autotuned lane scheduler: {tuning}")
-    net = NetDenoiser(ctx).load()
+    net = NetDenoiser(ctx).load(args.weights) if args.weights else NetDenoiser(ctx).load()
     post = PostProcessStack(ctx)
     pp_full = make_params(enable=VHS_PASSES, atrous=True, iterations=5)
     pp_denoise = make_params(atrous=True, iterations=5)

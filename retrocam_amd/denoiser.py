@@ -143,35 +143,97 @@ def _raw_from(acc, gbuf, galb, variance, w, h):
 
 
 def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256), spp_ref: int = 1024, log=None,
-                 group: int = N.RC_MAX_SCENE_SLOTS):
+                 group: int = N.RC_MAX_SCENE_SLOTS, levels: int = 1, seed: int = 1):
     """Pairs over workloads.training_scene(seed); spp_in cycles through the given
-    counts. Views are rendered in batches of `group` scenes, one persistent
-    launch per batch and sample chunk (rc_views_render)."""
+    counts. With levels > 1 each view yields `levels` inputs at increasing sample
+    counts drawn from spp_in (one accumulation read at each count) that share
+    its high-spp target. Views are rendered in batches of `group` scenes, one
+    persistent launch per batch and sample chunk (rc_views_render)."""
     from . import workloads
     from .batch import ViewBatch, ViewRequest
     seeds = list(seeds)
+    rng = np.random.default_rng(seed)
     vb = ViewBatch(ctx)
     raws, tgts = [], []
     for g0 in range(0, len(seeds), group):
         ws = [workloads.training_scene(int(s), size=size) for s in seeds[g0:g0 + group]]
         for k, w in enumerate(ws):
             vb.upload_scene(k, w.scene)
-        reqs = [ViewRequest(k, w.camera, w.params, int(spp_in[(g0 + k) % len(spp_in)])) for k, w in enumerate(ws)]
-        vb.render(reqs, reset=True)
-        for k in range(len(ws)):
-            raws.append(_raw_from(vb.read(k, N.RC_BUF_ACCUM), vb.read(k, N.RC_BUF_GBUFFER),
-                                  vb.read(k, N.RC_BUF_GALBEDO), vb.read(k, N.RC_BUF_VARIANCE), size, size))
-        done = 0
-        while done < spp_ref:
-            kk = min(512, spp_ref - done)
-            vb.render([ViewRequest(q.scene, q.camera, q.params, kk) for q in reqs], reset=done == 0, fp64=True)
-            done += kk
+        if levels <= 1:
+            lv = [[int(spp_in[(g0 + k) % len(spp_in)])] for k in range(len(ws))]
+        else:
+            lv = [sorted(int(v) for v in rng.choice(spp_in, size=levels, replace=False)) for _ in ws]
+        done = [0] * len(ws)
+        per_view = [[] for _ in ws]
+        for r in range(len(lv[0])):
+            reqs = [ViewRequest(k, w.camera, w.params, lv[k][r] - done[k]) for k, w in enumerate(ws)]
+            vb.render(reqs, reset=r == 0)
+            for k in range(len(ws)):
+                done[k] = lv[k][r]
+                per_view[k].append(_raw_from(vb.read(k, N.RC_BUF_ACCUM), vb.read(k, N.RC_BUF_GBUFFER),
+                                             vb.read(k, N.RC_BUF_GALBEDO), vb.read(k, N.RC_BUF_VARIANCE), size,
+                                             size))
+        done_ref = 0
+        while done_ref < spp_ref:
+            kk = min(512, spp_ref - done_ref)
+            vb.render([ViewRequest(k, w.camera, w.params, kk) for k, w in enumerate(ws)], reset=done_ref == 0,
+                      fp64=True)
+            done_ref += kk
             ctx.sync()
         for k in range(len(ws)):
             acc = vb.read(k, N.RC_BUF_ACCUM64)
-            tgts.append((acc[:, 0:3] / acc[:, 3:4]).astype(np.float32).reshape(size, size, 3))
+            t = (acc[:, 0:3] / acc[:, 3:4]).astype(np.float32).reshape(size, size, 3)
+            for raw in per_view[k]:
+                raws.append(raw)
+                tgts.append(t)
         if log:
             log(f"[dataset] {min(g0 + group, len(seeds))}/{len(seeds)} scenes")
+    return np.stack(raws), np.stack(tgts)
+
+
+def make_frame_crops(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256), spp_ref: int = 1024,
+                     levels: int = 3, log=None, seed: int = 2):
+    """Pairs from full frames of workloads.full_frame(seed) (854x480, 250k
+    triangles, thin lens), cut into size x size crops: `levels` inputs per frame
+    at increasing sample counts from spp_in, targets at spp_ref with FP64
+    accumulation."""
+    from . import workloads
+    from .renderer import Renderer, SceneUploader
+    rng = np.random.default_rng(seed)
+    raws, tgts = [], []
+    r = Renderer(ctx)
+    for i, fs in enumerate(seeds):
+        w = workloads.full_frame(seed=int(fs))
+        W, H = w.camera.width, w.camera.height
+        SceneUploader(ctx).upload(w.scene)
+        lv = sorted(int(v) for v in rng.choice(spp_in, size=levels, replace=False))
+        r.set_accum_mode(N.RC_ACCUM_FP32)
+        r.reset()
+        full = []
+        done = 0
+        for n in lv:
+            r.render(w.camera, w.params, n_samples=n - done)
+            done = n
+            full.append(_raw_from(r.read(N.RC_BUF_ACCUM), r.read(N.RC_BUF_GBUFFER), r.read(N.RC_BUF_GALBEDO),
+                                  r.read(N.RC_BUF_VARIANCE), W, H))
+        r.set_accum_mode(N.RC_ACCUM_FP64)
+        r.reset()
+        done = 0
+        while done < spp_ref:
+            kk = min(256, spp_ref - done)
+            r.render(w.camera, w.params, n_samples=kk)
+            done += kk
+            ctx.sync()
+        acc = r.read(N.RC_BUF_ACCUM64)
+        t = (acc[:, 0:3] / acc[:, 3:4]).astype(np.float32).reshape(H, W, 3)
+        r.set_accum_mode(N.RC_ACCUM_FP32)
+        for y0 in range(0, H - size + 1, size):
+            for x0 in range(0, W - size + 1, size):
+                for raw in full:
+                    raws.append(raw[y0:y0 + size, x0:x0 + size])
+                    tgts.append(t[y0:y0 + size, x0:x0 + size])
+        if log:
+            log(f"[dataset] full frame {i + 1}/{len(seeds)} (seed {fs})")
     return np.stack(raws), np.stack(tgts)
 
 

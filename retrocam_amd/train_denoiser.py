@@ -3,10 +3,14 @@
     python -m retrocam_amd.train_denoiser [--scenes 384] [--size 128] [--ref-spp 1024]
         [--steps 12000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
 
-Training data: seeded views from workloads.training_scene rendered by this
-package's path tracer (inputs at 4..256 spp cycling over the scenes, targets
-at --ref-spp with FP64 accumulation; the loss is minimised by the targets'
-mean, so their remaining noise averages out over the dataset). The optimizer runs entirely in
+Training data, rendered by this package's path tracer: seeded views from
+workloads.training_scene, each giving --levels inputs at increasing sample
+counts from 4..256 (cumulative reads of one accumulation) that share a target at
+--ref-spp with FP64 accumulation; and 128x128 crops of full frames of the
+benchmark scene generator (workloads.full_frame, seeds 2 .. 1 + --frame-seeds;
+seed 1, the benchmark frame, is never used) with the same input levels. The
+loss is minimised by the targets' mean, so their remaining noise averages out
+over the dataset. The optimizer runs entirely in
 libretrocam_hip.so (rc_net_train: forward, backward and Adam kernels).
 Evaluation: relMSE on held-out views (seeds >= 1,000,000) against their
 high-spp renders, for the noisy input, a-trous and the network.
@@ -23,7 +27,7 @@ import numpy as np
 from . import _native as N
 from . import workloads
 from .batch import ViewBatch, ViewRequest
-from .denoiser import WEIGHTS_PATH, NetDenoiser, make_dataset, rel_mse
+from .denoiser import WEIGHTS_PATH, NetDenoiser, make_dataset, make_frame_crops, rel_mse
 from .post import TemporalInputs, make_params
 from .renderer import RenderContext
 
@@ -60,6 +64,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--eval-scenes", type=int, default=16)
     ap.add_argument("--out", default=WEIGHTS_PATH)
+    ap.add_argument("--levels", type=int, default=3, help="inputs per training view (increasing spp)")
+    ap.add_argument("--frame-seeds", type=int, default=16,
+                    help="full-frame scenes (seeds 2..) cut into crops; seed 1 is the benchmark frame, never used")
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
     args = ap.parse_args(argv)
 
@@ -68,7 +75,13 @@ def main(argv=None):
 
     ctx = RenderContext(0)
     t0 = time.perf_counter()
-    raw, tgt = make_dataset(ctx, range(args.scenes), size=args.size, spp_ref=args.ref_spp, log=log)
+    raw, tgt = make_dataset(ctx, range(args.scenes), size=args.size, spp_ref=args.ref_spp, log=log,
+                            levels=args.levels, seed=args.seed)
+    if args.frame_seeds > 0:
+        fr, ft = make_frame_crops(ctx, range(2, 2 + args.frame_seeds), size=args.size, spp_ref=args.ref_spp,
+                                  levels=args.levels, log=log, seed=args.seed + 1)
+        raw, tgt = np.concatenate([raw, fr]), np.concatenate([tgt, ft])
+        del fr, ft
     t_data = time.perf_counter() - t0
     log(f"[train] dataset: {raw.shape[0]} views {args.size}x{args.size} in {t_data:.1f} s")
     net = NetDenoiser(ctx).init(args.seed)
@@ -92,7 +105,8 @@ def main(argv=None):
     ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.eval_ref_spp) \
         if args.eval_scenes > 0 else np.full((1, 3), np.nan)
     summary = {
-        "scenes": args.scenes, "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
+        "scenes": args.scenes, "levels": args.levels, "frame_seeds": args.frame_seeds, "pairs": int(raw.shape[0]),
+        "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
         "batch": args.batch, "lr": args.lr, "params": net.n,
         "dataset_s": round(t_data, 2), "train_s": round(t_train, 2),
         "ms_per_step": round(t_train / args.steps * 1e3, 3),
