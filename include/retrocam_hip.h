@@ -319,7 +319,8 @@ int32_t rc_net_param_count(void);
 int rc_net_init(rc_ctx *ctx, uint32_t seed);
 int rc_net_set_params(rc_ctx *ctx, const float *params, int32_t count);
 int rc_net_get_params(rc_ctx *ctx, float *params, int32_t count);
-/* Copies a training set to the device: raw n x h x w x 12, target n x h x w x 3. */
+/* Copies a training set to the device: raw n x h x w x 12, target n x h x w x 3;
+ * n x h x w <= 2^29 pixels (60 bytes each on the device). */
 int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32_t n, int32_t h, int32_t w);
 /* `steps` Adam steps (b1 0.9, b2 0.999) on batches of `batch` crops drawn
  * from the dataset with random flips; loss = mean (x-r)^2/(r^2+0.01). Writes

@@ -1047,7 +1047,12 @@ int rc_net_set_dataset(rc_ctx *ctx, const float *raw, const float *target, int32
     (void)hipSetDevice(ctx->device);
     Net *net = get_net(ctx);
     if (!net) return set_error(ctx, RC_ERR_OOM, "host allocation failed");
-    int rc = check_shape(ctx, n, h, w);
+    // The dataset holds 60 B per pixel (raw + target) and is indexed with size_t by
+    // make_batch; it is not bound by the forward workspace's 64 Mpixel limit.
+    if (n < 1) return set_error(ctx, RC_ERR_INVALID_ARG, "training set: n >= 1 required");
+    if ((size_t)n * h * w > ((size_t)512 << 20))
+        return set_error(ctx, RC_ERR_LIMIT, "training set larger than 512 Mpixels");
+    int rc = check_shape(ctx, 1, h, w);
     if (rc) return rc;
     if (!raw || !target) return set_error(ctx, RC_ERR_INVALID_ARG, "NULL dataset");
     hipError_t e = hipStreamSynchronize(ctx->stream);

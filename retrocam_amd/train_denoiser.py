@@ -1,7 +1,9 @@
 """Trains the learned denoiser on the GPU and writes its weights.
 
-    python -m retrocam_amd.train_denoiser [--scenes 384] [--size 128] [--ref-spp 1024]
-        [--steps 12000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
+    python -m retrocam_amd.train_denoiser [--scenes 1024] [--levels 4] [--frame-seeds 32]
+        [--ref-spp 2048] [--steps 60000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
+
+(the defaults are the settings of the committed weights)
 
 Training data, rendered by this package's path tracer: seeded views from
 workloads.training_scene, each giving --levels inputs at increasing sample
@@ -54,18 +56,18 @@ def evaluate(ctx, net, seeds, size, spp_in, ref_spp):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=384)
+    ap.add_argument("--scenes", type=int, default=1024)
     ap.add_argument("--size", type=int, default=128)
-    ap.add_argument("--ref-spp", type=int, default=1024)
+    ap.add_argument("--ref-spp", type=int, default=2048)
     ap.add_argument("--eval-ref-spp", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--eval-scenes", type=int, default=16)
     ap.add_argument("--out", default=WEIGHTS_PATH)
-    ap.add_argument("--levels", type=int, default=3, help="inputs per training view (increasing spp)")
-    ap.add_argument("--frame-seeds", type=int, default=16,
+    ap.add_argument("--levels", type=int, default=4, help="inputs per training view (increasing spp)")
+    ap.add_argument("--frame-seeds", type=int, default=32,
                     help="full-frame scenes (seeds 2..) cut into crops; seed 1 is the benchmark frame, never used")
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
     args = ap.parse_args(argv)
