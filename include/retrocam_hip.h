@@ -370,7 +370,7 @@ int rc_frame_release(rc_ctx *ctx);
 
 /* ---- tuning (no counterpart in the reference) --------------------------- */
 /* Launch parameters that change speed, never results. */
-#define RC_LANE_VARIANTS 9
+#define RC_LANE_VARIANTS 11
 typedef struct rc_tuning {
     int32_t laneThreshold;  /* RC_PT_LANES: a wave stops walking and shades once this
                                many of its 64 lanes wait (1..64, default 32)       */

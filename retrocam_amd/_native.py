@@ -39,7 +39,7 @@ RC_ACCUM_FP64 = 1
 RC_PT_MEGAKERNEL = 0
 RC_PT_WAVEFRONT = 1
 RC_PT_LANES = 2
-RC_LANE_VARIANTS = 9
+RC_LANE_VARIANTS = 11
 
 RC_DENOISE_ATROUS = 0
 RC_DENOISE_NET = 1

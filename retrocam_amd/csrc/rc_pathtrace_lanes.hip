@@ -215,6 +215,76 @@ __device__ __forceinline__ bool walk_step_tri(const SceneView &S, Walk &w, bool 
     return w.li >= w.le && w.top == 0;
 }
 
+// The walk with one triangle test AND one node operation per step: a lane tests
+// the next triangle of its pending leaf and, once that leaf has no triangle
+// left, pops or expands the next node in the same step. Leaves are tested in the
+// reference's order and each leaf's triangles in order; only the cull threshold
+// can be stale (not yet lowered by the pending leaf), which visits extra nodes
+// whose triangles all lie beyond the final closest hit (t >= entry > threshold
+// >= final t), so they can neither win nor tie. Any-hit walks do not depend on
+// the order at all. Lanes in leaves and lanes at inner nodes thus share steps.
+template <bool STATS, int NS>
+__device__ __forceinline__ bool walk_step_dual(const SceneView &S, Walk &w, bool anyHit, const StackRef &st,
+                                               unsigned long long *spill, Counters &C) {
+    if (w.li < w.le) {
+        float t, bu, bv;
+        if (STATS) C.triTests++;
+        const int i = w.li++;
+        if (tri_test(S, w.ro, w.rd, i, t, bu, bv) && t < w.ht) {
+            w.ht = t;
+            w.hbu = bu;
+            w.hbv = bv;
+            w.htri = i;
+            if (STATS) C.hitUpdates++;
+            if (anyHit) return true;
+        }
+    }
+    if (w.li >= w.le) {
+        if (!w.have) {
+            if (w.top == 0) return true;
+            --w.top;
+            const int2 e = stack_pop<NS>(st, spill, w.top);
+            w.ref = e.x;
+            w.tEntry = __int_as_float(e.y);
+            if (STATS) C.nodePops++;
+        }
+        w.have = false;
+        if (w.tEntry <= w.ht) {
+            if (w.ref < 0) {
+                int start, count;
+                leaf_range(S, w.ref, start, count);
+                w.li = start;
+                w.le = start + count;
+            } else if (w.top < 62) {
+                const float4 *n = S.inner + 4 * (size_t)w.ref;
+                const float4 a = n[0], b = n[1], c = n[2];
+                const int4 r = ((const int4 *)n)[3];
+                if (STATS) C.childTests++;
+                float tl, trr;
+                const bool hl = aabb(w.ro, w.inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
+                const bool hr = aabb(w.ro, w.inv, b.z, b.w, c.x, c.y, c.z, c.w, trr);
+                if (hl || hr) {
+                    if (STATS) C.nodePops++;
+                    w.have = true;
+                    if (hl && hr) {
+                        const bool nearLeft = tl < trr;
+                        const int2 far =
+                            nearLeft ? make_int2(r.y, __float_as_int(trr)) : make_int2(r.x, __float_as_int(tl));
+                        stack_push<NS>(st, spill, w.top, far);
+                        ++w.top;
+                        w.ref = nearLeft ? r.x : r.y;
+                        w.tEntry = nearLeft ? tl : trr;
+                    } else {
+                        w.ref = hl ? r.x : r.y;
+                        w.tEntry = hl ? tl : trr;
+                    }
+                }
+            }
+        }
+    }
+    return w.li >= w.le && !w.have && w.top == 0;
+}
+
 // Packed per-lane path flags: ray kind (2 bits), LCA channel (2), bounce (5),
 // lastDiffuse, and whether the path ends after the pending shadow ray.
 constexpr uint32_t F_LASTDIFF = 1u << 9, F_CONTEND = 1u << 10;
@@ -229,7 +299,7 @@ __device__ __forceinline__ uint32_t f_pack(int kind, int ch, int bounce, bool la
 // samples: [K][tilesX * tilesY * 64] float4 (r, g, b, 0) per unit.
 // cont: [3][gridDim.x * kBlock] float4, one slot per lane: the path's next ray
 // and NEE weight while its shadow ray is walked (kept out of registers).
-template <bool STATS, int NS, bool TRI>
+template <bool STATS, int NS, int WALK>
 __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restrict__ samples, int *counter,
                            int thresh, float4 *__restrict__ cont, const StackRef &st, unsigned long long *spill, Counters &C) {
     const SceneView &S = A.S;
@@ -448,8 +518,9 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
         const bool anyHit = (flags & 3u) == (uint32_t)RAY_SHADOW;
         while (true) {
             if (phase == PH_WALK) {
-                const bool done = TRI ? walk_step_tri<STATS, NS>(S, w, anyHit, st, spill, C)
-                                      : walk_step<STATS, NS>(S, w, anyHit, st, spill, C);
+                const bool done = WALK == 2   ? walk_step_dual<STATS, NS>(S, w, anyHit, st, spill, C)
+                                  : WALK == 1 ? walk_step_tri<STATS, NS>(S, w, anyHit, st, spill, C)
+                                              : walk_step<STATS, NS>(S, w, anyHit, st, spill, C);
                 if (done) phase = PH_SHADE;
             }
             if (__ballot(phase == PH_WALK) == 0ull) break;
@@ -460,14 +531,16 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
 
 // Kernel variants (rc_tuning.laneVariant): waves per SIMD W, LDS stack entries
 // NS (W workgroups x NS x 2 KB of stacks within a CU's 160 KB) and whether a
-// walk step tests a whole leaf or one triangle (TRI):
+// walk step tests a whole leaf (WALK 0), one triangle (1), or one triangle and
+// one node (2, walk_step_dual):
 //   0: W 7, NS 11            1: W 6, NS 13  2: W 8, NS 10  3: W 5, NS 16
 //   4: W 7, NS 11, TRI       5: W 6, NS 13, TRI       6: W 5, NS 16, TRI
 //   7: W 4, NS 20, TRI (default)                      8: W 3, NS 26, TRI
+//   9: W 4, NS 20, dual     10: W 5, NS 16, dual
 // On the 250k-triangle frame at 80 samples: variant 0 177 ms, 4 141 ms, 6 138 ms,
 // 7 121 ms (126 VGPRs, no register spills; the 5-wave and higher variants spill
 // around the shading block and move 5x the HBM bytes in scratch traffic), 8 144 ms.
-template <bool STATS, int W, int NS, bool TRI>
+template <bool STATS, int W, int NS, int WALK>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
 pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh, float4 *cont) {
     __shared__ int2 stk[NS][kBlock];
@@ -475,14 +548,14 @@ pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh, fl
     unsigned long long spill[64 - NS];
     Counters C;
     if (STATS) C = Counters{};
-    lanes_view<STATS, NS, TRI>(A, A.frame0, A.nSamples, samples, counter, thresh, cont, st, spill, C);
+    lanes_view<STATS, NS, WALK>(A, A.frame0, A.nSamples, samples, counter, thresh, cont, st, spill, C);
     if (STATS) wave_add_counters(A.counters, C);
 }
 
 // Many views in one launch: every wave works through the views in table order
 // (heaviest first), taking units of a view until its counter is exhausted. The
 // table is read in place (uniform addresses: scalar loads), not copied.
-template <bool STATS, int W, int NS, bool TRI>
+template <bool STATS, int W, int NS, int WALK>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
 pathtrace_lanes_views_kernel(const PassArgs *__restrict__ views, const LaneChunk *__restrict__ chunks, int nViews,
                              int thresh, float4 *cont, unsigned long long *counters) {
@@ -494,7 +567,7 @@ pathtrace_lanes_views_kernel(const PassArgs *__restrict__ views, const LaneChunk
     for (int v = 0; v < nViews; ++v) {
         const LaneChunk &ck = chunks[v];
         if (ck.nSamples <= 0) continue;
-        lanes_view<STATS, NS, TRI>(views[v], views[v].frame0 + ck.s0, ck.nSamples, ck.samples, ck.counter, thresh,
+        lanes_view<STATS, NS, WALK>(views[v], views[v].frame0 + ck.s0, ck.nSamples, ck.samples, ck.counter, thresh,
                                    cont, st, spill, C);
     }
     if (STATS) wave_add_counters(counters, C);
@@ -505,15 +578,17 @@ using LaneViewsKernel = void (*)(const PassArgs *, const LaneChunk *, int, int, 
 
 #define RC_LANE_VARIANT_TABLE(K, S)                                              \
     switch (variant) {                                                           \
-        case 1: return S ? K<true, 6, 13, false> : K<false, 6, 13, false>;       \
-        case 2: return S ? K<true, 8, 10, false> : K<false, 8, 10, false>;       \
-        case 3: return S ? K<true, 5, 16, false> : K<false, 5, 16, false>;       \
-        case 4: return S ? K<true, 7, 11, true> : K<false, 7, 11, true>;         \
-        case 5: return S ? K<true, 6, 13, true> : K<false, 6, 13, true>;         \
-        case 6: return S ? K<true, 5, 16, true> : K<false, 5, 16, true>;         \
-        case 7: return S ? K<true, 4, 20, true> : K<false, 4, 20, true>;         \
-        case 8: return S ? K<true, 3, 26, true> : K<false, 3, 26, true>;         \
-        default: return S ? K<true, 7, 11, false> : K<false, 7, 11, false>;      \
+        case 1: return S ? K<true, 6, 13, 0> : K<false, 6, 13, 0>;               \
+        case 2: return S ? K<true, 8, 10, 0> : K<false, 8, 10, 0>;               \
+        case 3: return S ? K<true, 5, 16, 0> : K<false, 5, 16, 0>;               \
+        case 4: return S ? K<true, 7, 11, 1> : K<false, 7, 11, 1>;               \
+        case 5: return S ? K<true, 6, 13, 1> : K<false, 6, 13, 1>;               \
+        case 6: return S ? K<true, 5, 16, 1> : K<false, 5, 16, 1>;               \
+        case 7: return S ? K<true, 4, 20, 1> : K<false, 4, 20, 1>;               \
+        case 8: return S ? K<true, 3, 26, 1> : K<false, 3, 26, 1>;               \
+        case 9: return S ? K<true, 4, 20, 2> : K<false, 4, 20, 2>;               \
+        case 10: return S ? K<true, 5, 16, 2> : K<false, 5, 16, 2>;              \
+        default: return S ? K<true, 7, 11, 0> : K<false, 7, 11, 0>;              \
     }
 
 LaneKernel lane_kernel(int variant, bool stats) { RC_LANE_VARIANT_TABLE(pathtrace_lanes_kernel, stats) }

@@ -1,7 +1,7 @@
 """Trains the learned denoiser on the GPU and writes its weights.
 
-    python -m retrocam_amd.train_denoiser [--scenes 1024] [--levels 4] [--frame-seeds 32]
-        [--ref-spp 2048] [--steps 60000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
+    python -m retrocam_amd.train_denoiser [--scenes 1024] [--levels 4] [--frame-seeds 64]
+        [--ref-spp 2048] [--steps 120000] [--batch 8] [--out retrocam_amd/weights/unet_denoiser.npy]
 
 (the defaults are the settings of the committed weights)
 
@@ -60,14 +60,14 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--ref-spp", type=int, default=2048)
     ap.add_argument("--eval-ref-spp", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=60000)
+    ap.add_argument("--steps", type=int, default=120000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--eval-scenes", type=int, default=16)
     ap.add_argument("--out", default=WEIGHTS_PATH)
     ap.add_argument("--levels", type=int, default=4, help="inputs per training view (increasing spp)")
-    ap.add_argument("--frame-seeds", type=int, default=32,
+    ap.add_argument("--frame-seeds", type=int, default=64,
                     help="full-frame scenes (seeds 2..) cut into crops; seed 1 is the benchmark frame, never used")
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
     args = ap.parse_args(argv)
