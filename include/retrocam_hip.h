@@ -278,6 +278,19 @@ int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t);
 int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
                   const rc_temporal *t);
 
+/* ---- frame export ------------------------------------------------------- */
+/* FrameExporter.exportFrame (src/main/java/com/retrocam/export/FrameExporter.java:37-65):
+ * the display image of the last rc_post (RGBA8, ACES + sRGB) with its rows flipped to
+ * top-down order on the device, read back once, and written to `path` as a PNG (8-bit
+ * RGBA) when path is not NULL. rgbaTopDown (may be NULL) receives the flipped pixels
+ * (width x height x 4 bytes). The reference's JPEG option is not provided. */
+int rc_export_frame(rc_ctx *ctx, const char *path, uint8_t *rgbaTopDown);
+/* Size of the display image of the last rc_post (RC_ERR_STATE before one). */
+int rc_display_size(const rc_ctx *ctx, int32_t *width, int32_t *height);
+/* The PNG writer alone (host only, no device): width x height RGBA8, rows bottom-up
+ * (GL order) when flipRows = 1, top-down otherwise. */
+int rc_encode_png(const uint8_t *rgba, int32_t width, int32_t height, int32_t flipRows, const char *path);
+
 /* ---- batched views (no counterpart in the reference) --------------------- */
 /* Serving many render requests at once: each view names a scene slot, a
  * camera (with its own resolution), path-trace switches and a sample count.

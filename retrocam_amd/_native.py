@@ -158,6 +158,9 @@ HIP_SYMBOLS = {
     "rc_net_grad": (C.c_int, [_VP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP, _FP]),
     "rc_net_forward": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.c_int32, _FP]),
     "rc_set_denoiser": (C.c_int, [_VP, C.c_int32]),
+    "rc_export_frame": (C.c_int, [_VP, C.c_char_p, C.POINTER(C.c_uint8)]),
+    "rc_display_size": (C.c_int, [_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "rc_encode_png": (C.c_int, [C.POINTER(C.c_uint8), C.c_int32, C.c_int32, C.c_int32, C.c_char_p]),
     "rc_frame_capture_begin": (C.c_int, [_VP]),
     "rc_frame_capture_end": (C.c_int, [_VP]),
     "rc_frame_replay": (C.c_int, [_VP, C.POINTER(RcTemporal)]),

@@ -76,6 +76,16 @@ class PostProcessStack:
         self.ctx.check(self._lib.rc_post(self.ctx.h, C.byref(params), C.byref(t)))
         return self._read() if read else None
 
+    def export_frame(self, path: str | None = None) -> np.ndarray:
+        """FrameExporter.exportFrame: the displayed frame, rows top-down (H, W, 4)
+        uint8, also written as a PNG to `path` when given."""
+        w, h = C.c_int32(), C.c_int32()
+        self.ctx.check(self._lib.rc_display_size(self.ctx.h, C.byref(w), C.byref(h)))
+        out = np.empty((h.value, w.value, 4), np.uint8)
+        self.ctx.check(self._lib.rc_export_frame(self.ctx.h, path.encode() if path else None,
+                                                 out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
     def run_on_image(self, rgba: np.ndarray, width: int, height: int, params: N.RcPostParams,
                      temporal: TemporalInputs, read: bool = True):
         img = np.ascontiguousarray(rgba, np.float32).reshape(height * width, 4)

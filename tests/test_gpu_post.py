@@ -82,3 +82,24 @@ def test_full_chain_on_rendered_frame(gpu_ctx, cornell):
     hdr, _ = rco.post_chain(acc, w.camera.width, w.camera.height, w.spp, p, TemporalInputs().to_c(), None, None,
                             None)
     check(g.hdr, hdr)
+
+
+def test_export_frame_matches_display(gpu_ctx, cornell, tmp_path):
+    # FrameExporter.exportFrame: the display image, flipped to top-down rows, as a PNG.
+    from retrocam_amd.post import VHS_PASSES, PostProcessStack, TemporalInputs, make_params
+    from retrocam_amd.renderer import Renderer, SceneUploader
+    from retrocam_amd.scene import CameraSpec
+    from test_export_cpu import decode_png
+    w = cornell
+    cam = CameraSpec(**{**w.camera.__dict__, "width": 96, "height": 54})
+    SceneUploader(gpu_ctx).upload(w.scene)
+    r = Renderer(gpu_ctx)
+    r.reset()
+    r.render(cam, w.params, n_samples=4)
+    post = PostProcessStack(gpu_ctx)
+    out = post.run_on_accum(make_params(enable=VHS_PASSES, atrous=True, iterations=3), TemporalInputs(0.5, 1.0, 2))
+    path = str(tmp_path / "frame.png")
+    top_down = post.export_frame(path)
+    disp = out.rgba8.reshape(54, 96, 4)
+    assert np.array_equal(top_down, disp[::-1])
+    assert np.array_equal(decode_png(open(path, "rb").read()), disp[::-1])
