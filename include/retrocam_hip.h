@@ -379,6 +379,12 @@ int rc_set_denoiser(rc_ctx *ctx, int32_t mode);
 int rc_frame_capture_begin(rc_ctx *ctx);
 int rc_frame_capture_end(rc_ctx *ctx);
 int rc_frame_replay(rc_ctx *ctx, const rc_temporal *t);
+/* Camera of the recorded single-view path-trace launches and SPPM gathers for
+ * the following replays (a video's moving or refocusing camera, e.g.
+ * TemporalState.focalDist changing per frame, TemporalState.java:96-111); its
+ * resolution must be the recorded one. NULL returns to the recorded cameras.
+ * Views keep the cameras of their tables. */
+int rc_frame_set_camera(rc_ctx *ctx, const rc_camera *cam);
 int rc_frame_release(rc_ctx *ctx);
 
 /* ---- tuning (no counterpart in the reference) --------------------------- */

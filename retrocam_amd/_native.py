@@ -112,6 +112,17 @@ class RcsInfo(C.Structure):
     ]
 
 
+class RctSettings(C.Structure):
+    _fields_ = [("focusDistM", _F), ("afSpeed", _F), ("agcMaxGain", _F), ("agcSpeed", _F)]
+
+
+class RctState(C.Structure):
+    _fields_ = [("agcGain", _F), ("focalDist", _F), ("time", _F), ("agcGainTarget", _F),
+                ("focalDistTarget", _F), ("whiteBalance", _F * 9)]
+
+
+RCK_STEP, RCK_LINEAR, RCK_BEZIER = 0, 1, 2
+
 _FP = C.POINTER(C.c_float)
 _VP = C.c_void_p
 
@@ -165,6 +176,7 @@ HIP_SYMBOLS = {
     "rc_frame_capture_end": (C.c_int, [_VP]),
     "rc_frame_replay": (C.c_int, [_VP, C.POINTER(RcTemporal)]),
     "rc_frame_release": (C.c_int, [_VP]),
+    "rc_frame_set_camera": (C.c_int, [_VP, C.POINTER(RcCamera)]),
     "rc_get_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_set_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_autotune": (C.c_int, [_VP, C.POINTER(RcCamera), C.POINTER(RcPtParams), C.c_int32, C.POINTER(RcTuning),
@@ -193,6 +205,23 @@ HOST_SYMBOLS = {
     "rcs_orbit_basis": (None, [C.c_float] * 6 + [_FP] * 4),
     "rcs_lookat_basis": (None, [_FP] * 5),
     "rcs_tan_half_fov": (C.c_float, [C.c_float]),
+    "rct_init": (None, [C.POINTER(RctState)]),
+    "rct_update": (None, [C.POINTER(RctState), C.c_float, C.POINTER(RctSettings)]),
+    "rct_reset_for_video": (None, [C.POINTER(RctState), C.POINTER(RctSettings)]),
+    "rct_update_for_video_frame": (None, [C.POINTER(RctState), C.c_float, C.c_float, C.POINTER(RctSettings)]),
+    "rct_run_video": (C.c_int32, [C.POINTER(RctState), C.POINTER(RctSettings), _FP, C.c_float, C.c_int32,
+                                  _FP, _FP, _FP]),
+    "rcj_frame_dt": (C.c_float, [C.c_float]),
+    "rcj_total_frames": (C.c_int32, [C.c_float, C.c_float]),
+    "rcj_canonical_time": (C.c_float, [C.c_int32, C.c_float]),
+    "rck_new": (C.c_void_p, []),
+    "rck_free": (None, [_VP]),
+    "rck_add": (C.c_int32, [_VP, C.c_float, C.c_float, C.c_int32]),
+    "rck_remove": (C.c_int32, [_VP, C.c_int32]),
+    "rck_count": (C.c_int32, [_VP]),
+    "rck_get": (C.c_int32, [_VP, C.c_int32, _FP, _FP, C.POINTER(C.c_int32)]),
+    "rck_evaluate": (C.c_float, [_VP, C.c_float, C.c_float]),
+    "rck_evaluate_many": (C.c_int32, [_VP, _FP, C.c_int32, C.c_float, _FP]),
 }
 
 

@@ -359,6 +359,7 @@ int rc_views_render(rc_ctx *ctx, const rc_view *views, int32_t n, int32_t flags)
         PassArgs &A = T.args[k];
         A.S = scene_view(*scene_slot(ctx, v.scene));
         A.cam = cam_args(v.cam);
+        A.dcam = &V->dArgs[k].cam;
         A.nee = v.params.neeEnabled;
         A.neeClamp = v.params.neeFireflyClamp;
         A.maxBounces = v.params.maxBounces;

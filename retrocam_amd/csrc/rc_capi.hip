@@ -34,8 +34,35 @@ int capture_block(rc_ctx *ctx, const char *what) {
                                          "for the device or copy to the host); run the frame once before capturing it");
 }
 
+__global__ void fill_cam_kernel(CamArgs *p, int n, CamArgs c) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = c;
+}
+
+int camera_slot(rc_ctx *ctx, const CamArgs &c, const CamArgs **out) {
+    *out = nullptr;
+    if (ctx->capturing) {
+        if (ctx->nCams >= kMaxCameraSlots)
+            return set_error(ctx, RC_ERR_LIMIT, "a captured frame records at most 4096 path-trace launches and SPPM "
+                                                "gathers (camera slots)");
+        ctx->hCams[ctx->nCams] = c;
+        *out = ctx->dCams + ctx->nCams++;
+        return RC_OK;
+    }
+    CamArgs *slot = ctx->dCams + kMaxCameraSlots;
+    hipLaunchKernelGGL(fill_cam_kernel, dim3(1), dim3(64), 0, ctx->stream, slot, 1, c);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "camera upload");
+    *out = slot;
+    return RC_OK;
+}
+
 const rc_temporal *temporal_slot(rc_ctx *ctx, const rc_temporal &t) {
-    if (!ctx->capturing || ctx->nTemporal >= kMaxTemporalSlots) return nullptr;
+    if (!ctx->capturing) return nullptr;
+    if (ctx->nTemporal >= kMaxTemporalSlots) {
+        ctx->slotsExhausted = 1;  // later post chains keep their recorded inputs
+        return nullptr;
+    }
     ctx->hTemporal[ctx->nTemporal] = t;
     return ctx->dTemporal + ctx->nTemporal++;
 }
@@ -278,9 +305,14 @@ int rc_create(int device, rc_ctx **out) {
     e = hipMalloc((void **)&ctx->dTemporal, sizeof(rc_temporal) * kMaxTemporalSlots);
     if (e == hipSuccess)
         e = hipHostMalloc((void **)&ctx->hTemporal, sizeof(rc_temporal) * kMaxTemporalSlots, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&ctx->dCams, sizeof(CamArgs) * (kMaxCameraSlots + 1));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&ctx->hCams, sizeof(CamArgs) * kMaxCameraSlots, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipFree(ctx->counters);
         if (ctx->dTemporal) (void)hipFree(ctx->dTemporal);
+        if (ctx->hTemporal) (void)hipHostFree(ctx->hTemporal);
+        if (ctx->dCams) (void)hipFree(ctx->dCams);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return set_error(nullptr, RC_ERR_OOM, "allocating the temporal-input slots failed");
@@ -303,6 +335,8 @@ int rc_destroy(rc_ctx *ctx) {
     if (ctx->frameGraph) (void)hipGraphDestroy(ctx->frameGraph);
     if (ctx->dTemporal) (void)hipFree(ctx->dTemporal);
     if (ctx->hTemporal) (void)hipHostFree(ctx->hTemporal);
+    if (ctx->dCams) (void)hipFree(ctx->dCams);
+    if (ctx->hCams) (void)hipHostFree(ctx->hCams);
     free_scene(ctx->scene);
     for (auto &d : ctx->extraScenes) free_scene(d);
     free_views(ctx);
@@ -498,13 +532,16 @@ static int render_common(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params *
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_images(ctx, cam->width, cam->height))) return rc;
     hipError_t e;
+    const CamArgs *dcam = nullptr;
+    if (ctx->ptMode != RC_PT_WAVEFRONT && (rc = camera_slot(ctx, cam_args(*cam), &dcam))) return rc;
     if (ctx->ptMode == RC_PT_WAVEFRONT) {
         e = launch_wavefront(ctx, *cam, *p, n);
     } else if (ctx->ptMode == RC_PT_LANES && lanes_supported(ctx->scene)) {
         PassArgs A = pass_args(ctx, *cam, *p, n);
+        A.dcam = dcam;
         e = launch_lanes(ctx, A, ctx->laneThresh, ctx->laneVariant);
     } else {
-        e = launch_pathtrace(ctx, *cam, *p, n);
+        e = launch_pathtrace(ctx, *cam, *p, n, dcam);
     }
     if (e != hipSuccess) return hip_fail(ctx, e, "pathtrace launch");
     ctx->totalSamples += n;
@@ -766,6 +803,9 @@ int rc_frame_capture_begin(rc_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     release_frame(ctx);
     ctx->nTemporal = 0;
+    ctx->nCams = 0;
+    ctx->replayCamSet = 0;
+    ctx->slotsExhausted = 0;
     ctx->captureResetSeen = 0;
     ctx->captureViewsResetSeen = 0;
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
@@ -817,6 +857,8 @@ int rc_frame_replay(rc_ctx *ctx, const rc_temporal *t) {
         return set_error(ctx, RC_ERR_STATE,
                          "the view tables changed since the frame was captured (other views were rendered): capture "
                          "it again");
+    if (t && ctx->slotsExhausted)
+        return set_error(ctx, RC_ERR_LIMIT, "the captured frame has more post chains than temporal slots");
     (void)hipSetDevice(ctx->device);
     hipError_t e = hipSuccess;
     if (ctx->nTemporal > 0) {
@@ -830,6 +872,17 @@ int rc_frame_replay(rc_ctx *ctx, const rc_temporal *t) {
         }
         if (e != hipSuccess) return hip_fail(ctx, e, "temporal inputs of the captured frame");
     }
+    if (ctx->nCams > 0) {
+        if (ctx->replayCamSet) {
+            hipLaunchKernelGGL(fill_cam_kernel, dim3((ctx->nCams + 255) / 256), dim3(256), 0, ctx->stream,
+                               ctx->dCams, ctx->nCams, ctx->replayCam);
+            e = hipGetLastError();
+        } else {
+            e = hipMemcpyAsync(ctx->dCams, ctx->hCams, sizeof(CamArgs) * ctx->nCams, hipMemcpyHostToDevice,
+                               ctx->stream);
+        }
+        if (e != hipSuccess) return hip_fail(ctx, e, "camera of the captured frame");
+    }
     e = hipGraphLaunch(ctx->frameExec, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
     ctx->totalSamples = ctx->captured.totalSamples;
@@ -839,6 +892,24 @@ int rc_frame_replay(rc_ctx *ctx, const rc_temporal *t) {
     ctx->sppm.photonCount = ctx->captured.photonCount;
     ctx->sppm.hashSize = ctx->captured.hashSize;
     views_set_totals(ctx, ctx->captured.viewTotals);
+    return RC_OK;
+}
+
+int rc_frame_set_camera(rc_ctx *ctx, const rc_camera *cam) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (ctx->capturing) return set_error(ctx, RC_ERR_STATE, "rc_frame_set_camera while capturing");
+    if (!ctx->frameExec) return set_error(ctx, RC_ERR_STATE, "no captured frame");
+    if (!cam) {
+        ctx->replayCamSet = 0;
+        return RC_OK;
+    }
+    int rc;
+    if ((rc = validate_camera(ctx, cam))) return rc;
+    for (int i = 0; i < ctx->nCams; ++i)
+        if (ctx->hCams[i].width != cam->width || ctx->hCams[i].height != cam->height)
+            return set_error(ctx, RC_ERR_INVALID_ARG, "rc_frame_set_camera: the recorded frame has another resolution");
+    ctx->replayCam = cam_args(*cam);
+    ctx->replayCamSet = 1;
     return RC_OK;
 }
 

@@ -13,6 +13,15 @@
 
 namespace rc {
 
+// Camera + lens arguments of a path-trace launch (rc_camera in device form).
+struct CamArgs {
+    float pos[3], right[3], up[3], fwd[3];
+    float focalDist, lensRadius, tanHalf, saStrength;
+    float lca[3];
+    int blades, width, height;
+};
+CamArgs cam_args(const rc_camera &cam);  // rc_pathtrace.hip
+
 // Device scene (see rc_device.h for the layout).
 struct DeviceScene {
     float4 *inner = nullptr;
@@ -161,6 +170,12 @@ struct rc_ctx {
     uint64_t freeGen = 0;
     uint64_t graphFreeGen = 0;
     int captureViewsResetSeen = 0;
+    rc::CamArgs *dCams = nullptr;  // kMaxCameraSlots capture slots + 1 eager slot
+    rc::CamArgs *hCams = nullptr;  // pinned: the cameras recorded at capture
+    int nCams = 0;
+    int replayCamSet = 0;          // rc_frame_set_camera: replays use replayCam
+    rc::CamArgs replayCam{};
+    int slotsExhausted = 0;        // capture ran out of temporal slots
     // host-side state at the end of the recorded calls, restored by each replay
     struct {
         int32_t totalSamples = 0;
@@ -173,8 +188,10 @@ struct rc_ctx {
 
 namespace rc {
 
-// Post chains one captured frame can record (one temporal slot each).
+// Post chains one captured frame can record (one temporal slot each), and
+// camera-using launches (path trace, SPPM gather; one camera slot each).
 constexpr int kMaxTemporalSlots = 4096;
+constexpr int kMaxCameraSlots = 4096;
 
 int set_error(rc_ctx *ctx, int code, const std::string &msg);
 // RC_ERR_STATE if a frame is being captured: `what` would allocate, wait for
@@ -184,10 +201,16 @@ int capture_block(rc_ctx *ctx, const char *what);
 // capturing, the next device slot, which rc_frame_replay refills before each
 // launch.
 const rc_temporal *temporal_slot(rc_ctx *ctx, const rc_temporal &t);
+// The device copy of a launch's camera, in *out. Eagerly, the context's eager
+// slot, written by a fill kernel on the stream before the launch; while
+// capturing, the next camera slot, which rc_frame_replay refills (with the
+// recorded camera, or rc_frame_set_camera's). Returns an RC_* code.
+int camera_slot(rc_ctx *ctx, const CamArgs &c, const CamArgs **out);
 int hip_fail(rc_ctx *ctx, hipError_t e, const char *what);
 
 // Launches nSamples progressive passes starting at frame ctx->totalSamples.
-hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);
+hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples,
+                            const CamArgs *dcam);
 
 // Releases the SPPM buffers (rc_sppm.hip).
 void free_sppm(Sppm &s);

@@ -26,7 +26,7 @@ namespace rc {
 template <bool STATS, bool FP64>
 __device__ __forceinline__ void trace_pixel(const PassArgs &A, int px, int py, const StackRef &st, Counters &C) {
     const SceneView &S = A.S;
-    const CamArgs &cam = A.cam;
+    const CamArgs &cam = launch_camera(A);
     if (px < cam.width && py < cam.height) {
         const size_t pi = (size_t)py * (size_t)cam.width + (size_t)px;
         const bool lca = (fabsf(cam.lca[0]) + fabsf(cam.lca[2])) > 1e-4f;
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 
             if (tileOff[mid] <= t) lo = mid;
             else hi = mid;
         }
-        const PassArgs A = views[lo];
+        const PassArgs &A = views[lo];
         const int lt = t - tileOff[lo];
         const int tilesX = (A.cam.width + 15) >> 4;
         const int px = (lt % tilesX) * 16 + (wave & 1) * 8 + (lane & 7);
@@ -250,11 +250,14 @@ PassArgs pass_args(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int
     A.gbufId = ctx->img.gbufId;
     A.counters = ctx->counters;
     A.tileShift = ctx->laneTileShift;
+    A.dcam = nullptr;  // the caller sets it (camera_slot)
     return A;
 }
 
-hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples) {
-    const PassArgs A = pass_args(ctx, cam, p, nSamples);
+hipError_t launch_pathtrace(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples,
+                            const CamArgs *dcam) {
+    PassArgs A = pass_args(ctx, cam, p, nSamples);
+    A.dcam = dcam;
     dim3 grid((cam.width + 15) / 16, (cam.height + 15) / 16);
     const bool stats = ctx->countersEnabled && ctx->counters;
     const bool fp64 = ctx->accumMode == RC_ACCUM_FP64;

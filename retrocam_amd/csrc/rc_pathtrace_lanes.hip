@@ -303,7 +303,7 @@ template <bool STATS, int NS, int WALK>
 __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restrict__ samples, int *counter,
                            int thresh, float4 *__restrict__ cont, const StackRef &st, unsigned long long *spill, Counters &C) {
     const SceneView &S = A.S;
-    const CamArgs &cam = A.cam;
+    const CamArgs &cam = launch_camera(A);
     const int lane = threadIdx.x & 63;
     const bool lca = (fabsf(cam.lca[0]) + fabsf(cam.lca[2])) > 1e-4f;
     const LaneTile tl(cam.width, cam.height, A.tileShift);

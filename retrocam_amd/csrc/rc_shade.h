@@ -10,13 +10,6 @@
 
 namespace rc {
 
-struct CamArgs {
-    float pos[3], right[3], up[3], fwd[3];
-    float focalDist, lensRadius, tanHalf, saStrength;
-    float lca[3];
-    int blades, width, height;
-};
-
 // One progressive launch: nSamples passes from frame0 on, over the images given.
 struct PassArgs {
     SceneView S;
@@ -35,7 +28,19 @@ struct PassArgs {
     int *gbufId;
     unsigned long long *counters;
     int tileShift;  // lane scheduler: unit tiles of (1 << tileShift) x (64 >> tileShift) pixels
+    // The camera the kernels read: a device copy of `cam` (a camera slot of the
+    // context, or the entry of a device view table), never null.
+    const CamArgs *dcam;
 };
+
+// The camera of a launch. Its device copy is written before the launch and
+// never during it, so it is addressed through the constant address space, as
+// kernel arguments are: its words are invariant scalar loads that the compiler
+// reissues at each use instead of keeping 24 values live across the path loop.
+__device__ __forceinline__ const CamArgs &launch_camera(const PassArgs &A) {
+    typedef const __attribute__((address_space(4))) CamArgs *ConstCamPtr;
+    return *(const CamArgs *)(ConstCamPtr)A.dcam;
+}
 
 // Unit tiles of the lane scheduler: 64 pixels each, tw = 1 << shift wide and
 // th = 64 >> shift high; a view's units are numbered tile by tile.
@@ -303,7 +308,6 @@ __device__ __forceinline__ bool shade_vertex(const SceneView &S, int neeEnabled,
     return false;
 }
 
-CamArgs cam_args(const rc_camera &cam);  // rc_pathtrace.hip
 // Launch arguments of nSamples passes from frame ctx->totalSamples on the
 // context's scene and images (rc_pathtrace.hip).
 PassArgs pass_args(rc_ctx *ctx, const rc_camera &cam, const rc_pt_params &p, int nSamples);

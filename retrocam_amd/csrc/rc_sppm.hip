@@ -126,8 +126,7 @@ __global__ void __launch_bounds__(kBlock) photon_trace_kernel(TraceArgs A) {
 
 struct GatherArgs {
     SceneView S;
-    float pos[3], right[3], up[3], fwd[3];
-    float tanHalf;
+    const CamArgs *cam;  // device copy of the camera: position, basis, tanHalf
     int width, height, hashSize;
     float radius;
     const float4 *photons;
@@ -149,10 +148,11 @@ __global__ void __launch_bounds__(kBlock) photon_gather_kernel(GatherArgs A) {
     float aspect = (float)A.width / (float)A.height;
     float nx = (((float)x + 0.5f) / (float)A.width) * 2.0f - 1.0f;
     float ny = (((float)y + 0.5f) / (float)A.height) * 2.0f - 1.0f;
-    v3 F = mk(A.fwd[0], A.fwd[1], A.fwd[2]), R = mk(A.right[0], A.right[1], A.right[2]);
-    v3 U = mk(A.up[0], A.up[1], A.up[2]);
-    v3 rd = normalize((F + R * ((nx * aspect) * A.tanHalf)) + U * (ny * A.tanHalf));
-    v3 ro = mk(A.pos[0], A.pos[1], A.pos[2]);
+    const CamArgs &c = *A.cam;
+    v3 F = mk(c.fwd[0], c.fwd[1], c.fwd[2]), R = mk(c.right[0], c.right[1], c.right[2]);
+    v3 U = mk(c.up[0], c.up[1], c.up[2]);
+    v3 rd = normalize((F + R * ((nx * aspect) * c.tanHalf)) + U * (ny * c.tanHalf));
+    v3 ro = mk(c.pos[0], c.pos[1], c.pos[2]);
     bool found = false;
     v3 hp = mk(0, 0, 0);
     float4 alb = make_float4(0, 0, 0, 0);
@@ -330,11 +330,6 @@ int rc_sppm_gather(rc_ctx *ctx, const rc_camera *cam) {
     (void)hipSetDevice(ctx->device);
     GatherArgs A;
     A.S = scene_view(ctx->scene);
-    for (int k = 0; k < 3; ++k) {
-        A.pos[k] = cam->camPos[k]; A.right[k] = cam->camRight[k];
-        A.up[k] = cam->camUp[k]; A.fwd[k] = cam->camForward[k];
-    }
-    A.tanHalf = cam->fovTanHalf;
     A.width = cam->width;
     A.height = cam->height;
     A.hashSize = s.hashSize;
@@ -344,6 +339,7 @@ int rc_sppm_gather(rc_ctx *ctx, const rc_camera *cam) {
     A.next = s.next;
     A.accum = ctx->img.accum;
     A.accum64 = ctx->img.accum64;
+    if (int rc = camera_slot(ctx, cam_args(*cam), &A.cam)) return rc;
     dim3 grid((cam->width + 15) / 16, (cam->height + 15) / 16);
     if (ctx->accumMode == RC_ACCUM_FP64)
         hipLaunchKernelGGL(photon_gather_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, A);

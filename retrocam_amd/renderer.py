@@ -75,6 +75,15 @@ class RenderContext:
             t = temporal.to_c() if hasattr(temporal, "to_c") else temporal
         self.check(self._lib.rc_frame_replay(self.h, C.byref(t) if t is not None else None))
 
+    def set_frame_camera(self, camera=None):
+        """rc_frame_set_camera: the camera (CameraSpec) later replays use; None
+        restores the recorded one."""
+        if camera is None:
+            self.check(self._lib.rc_frame_set_camera(self.h, None))
+        else:
+            c = camera.to_c()
+            self.check(self._lib.rc_frame_set_camera(self.h, C.byref(c)))
+
     def release_frame(self):
         self.check(self._lib.rc_frame_release(self.h))
 
