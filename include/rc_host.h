@@ -77,6 +77,69 @@ void rcs_lookat_basis(const float *eye, const float *target,
 /* (float)tan(atan(0.012 / (focalLengthMm / 1000))) */
 float rcs_tan_half_fov(float focalLengthMm);
 
+/* ---- per-video-frame inputs (rc_temporal.cpp) ---------------------------
+ * The values the render path takes per video frame (rc_temporal, the camera's
+ * focalDist) come from these Java host objects, restated with binary32
+ * semantics:
+ *   TemporalState   src/main/java/com/retrocam/core/TemporalState.java:13-111
+ *   KeyframeTrack   src/main/java/com/retrocam/keyframe/KeyframeTrack.java:20-178
+ *   RenderJob timing src/main/java/com/retrocam/export/RenderJob.java:34-41 and
+ *                   RenderPipeline.java:150-151 (t = frame * dt) */
+
+/* RenderSettings fields the filters read (RenderSettings.java:28-43; defaults
+ * 2.0, 3.0, 6.0, 1.5). */
+typedef struct rct_settings {
+    float focusDistM;
+    float afSpeed;
+    float agcMaxGain;
+    float agcSpeed;
+} rct_settings;
+
+/* TemporalState fields, including the private IIR targets. */
+typedef struct rct_state {
+    float agcGain;
+    float focalDist;
+    float time;
+    float agcGainTarget;    /* setAgcGainTarget */
+    float focalDistTarget;  /* setFocalDistTarget */
+    float whiteBalance[9];  /* row-major 3x3, identity */
+} rct_state;
+
+void rct_init(rct_state *s);                                               /* field initialisers :16-29 */
+void rct_update(rct_state *s, float dt, const rct_settings *st);           /* update :44-64 */
+void rct_reset_for_video(rct_state *s, const rct_settings *st);            /* resetForVideoRendering :77-84 */
+void rct_update_for_video_frame(rct_state *s, float canonicalTime, float frameDt,
+                                const rct_settings *st);                   /* updateForVideoFrame :96-111 */
+/* nFrames video frames from frame 0 (RenderPipeline.tickVideoFrame :147-171):
+ * per frame, focalDistTarget := focusTargets[i] (when not NULL), then
+ * updateForVideoFrame(i * dt, dt) with dt = rcj_frame_dt(fps); writes the
+ * resulting time, agcGain and focalDist per frame (any output may be NULL).
+ * Returns nFrames, or -1. */
+int32_t rct_run_video(rct_state *s, const rct_settings *st, const float *focusTargets, float fps, int32_t nFrames,
+                      float *outTime, float *outAgcGain, float *outFocalDist);
+
+float rcj_frame_dt(float fps);                                  /* RenderJob.getFrameDt */
+int32_t rcj_total_frames(float durationSeconds, float fps);     /* RenderJob.getTotalFrames (video) */
+float rcj_canonical_time(int32_t frame, float frameDt);         /* frame * dt in binary32 */
+
+/* Keyframe tracks: keys kept in ascending time; interpolation per key. */
+typedef struct rck_track rck_track;
+enum { RCK_STEP = 0, RCK_LINEAR = 1, RCK_BEZIER = 2 };  /* BEZIER = Catmull-Rom */
+
+rck_track *rck_new(void);
+void rck_free(rck_track *t);
+/* addKeyframe: overwrites a key within 0.001 s, else sorted insert; interp is
+ * clamped to 0..2 (Interpolation.fromIndex). Returns the key's index, or -1. */
+int32_t rck_add(rck_track *t, float time, float value, int32_t interp);
+/* removeKeyframe: out-of-range indices are ignored. Returns the key count, or -1. */
+int32_t rck_remove(rck_track *t, int32_t index);
+int32_t rck_count(const rck_track *t);
+int32_t rck_get(const rck_track *t, int32_t index, float *time, float *value, int32_t *interp);
+/* evaluate(time, defaultValue): clamped outside the keys, STEP / LINEAR /
+ * Catmull-Rom inside. */
+float rck_evaluate(const rck_track *t, float time, float defaultValue);
+int32_t rck_evaluate_many(const rck_track *t, const float *times, int32_t n, float defaultValue, float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,7 +65,8 @@ def build_host() -> str:
     out = os.path.join(LIBDIR, "librc_host.so")
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-          f"-I{INC}", os.path.join(CSRC, "host", "rc_host.cpp"), "-o", tmp])
+          f"-I{INC}", os.path.join(CSRC, "host", "rc_host.cpp"),
+          os.path.join(CSRC, "host", "rc_temporal.cpp"), "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -9,7 +9,7 @@ import pytest
 from retrocam_amd import _native as N
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][\w]*\s*\**\s*\b((?:rc|rcs)_[a-z0-9_]+)\s*\(", re.M)
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][\w]*\s*\**\s*\b((?:rc|rcs|rct|rcj|rck)_[a-z0-9_]+)\s*\(", re.M)
 
 
 def declared(header):
