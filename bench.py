@@ -304,22 +304,32 @@ def sppm_leg(ctx, iters, log):
     return out
 
 
-def read_traffic(pt_mode):
+def read_traffic(pt_mode, tuning):
     """HBM bytes per sample of a full-frame path-trace launch, from the committed
     rocprofv3 PMC summary of that kernel (FETCH_SIZE + WRITE_SIZE scaled by a copy
-    kernel of known size, divided by the launch's samples)."""
+    kernel of known size, divided by the launch's samples). Only when this run's
+    tuning is the profiled one (spilling variants move several times more bytes);
+    otherwise None."""
     name = {"lanes": "pathtrace_lanes_kernel__full_frame__pmc.md",
             "megakernel": "pathtrace_kernel__full_frame__pmc.md"}.get(pt_mode)
     if name is None:
         return None
     path = os.path.join(ROOT, "profiles", name)
+    value, profiled = None, None
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_spp:"):
-                return float(line.split(":", 1)[1])
+                value = float(line.split(":", 1)[1])
+            elif line.startswith("traffic_tuning:"):
+                profiled = {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
     except OSError:
-        pass
-    return None
+        return None
+    if pt_mode == "lanes" and tuning is not None and profiled is not None:
+        mine = {"variant": tuning["laneVariant"], "threshold": tuning["laneThreshold"],
+                "tile_width": tuning["laneTileWidth"]}
+        if mine != profiled:
+            return None
+    return value
 
 
 def rel_mse(x, r):
@@ -377,6 +387,8 @@ def main():
     if args.pt_mode == "lanes" and not args.no_autotune:
         tuning = r.autotune(w.camera, w.params, samples=8)
         log(f"[bench] autotuned lane scheduler: {tuning}")
+    elif args.pt_mode == "lanes":
+        tuning = r.get_tuning()
     net = NetDenoiser(ctx).load(args.weights) if args.weights else NetDenoiser(ctx).load()
     post = PostProcessStack(ctx)
     pp_full = make_params(enable=VHS_PASSES, atrous=True, iterations=5)
@@ -538,7 +550,7 @@ def main():
     # the form that ran (the autotuner may choose the megakernel)
     ran = {N.RC_PT_LANES: "lanes", N.RC_PT_MEGAKERNEL: "megakernel", N.RC_PT_WAVEFRONT: "wavefront"}[
         tuning["ptMode"] if tuning else pt_mode]
-    traffic_spp = read_traffic(ran)
+    traffic_spp = read_traffic(ran, tuning)
     out = {
         "metric": "854x480 frames/s at relMSE<=0.01 vs 4096spp (net vs a-trous); Mrays/s",
         "value": round(fps, 4),

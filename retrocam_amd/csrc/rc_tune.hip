@@ -3,7 +3,10 @@
 // fastest. Every candidate computes the same image (the variants differ in
 // occupancy, LDS stack depth, triangles per walk step, the wait threshold that
 // starts a shading round, and the shape of the 64-pixel unit tiles), so
-// tuning changes speed only. Project-new: the reference has no counterpart.
+// tuning changes speed only. The search starts from the context's tuning and
+// moves only for a gain above kMinGain (2%): candidates within timing noise of
+// each other do not change the choice from run to run. Project-new: the reference
+// has no counterpart.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -19,7 +22,14 @@ namespace {
 struct Candidate {
     int variant, threshold, tileShift;
     int32_t mode;  // RC_PT_LANES, or RC_PT_MEGAKERNEL
+    bool operator==(const Candidate &o) const {
+        return variant == o.variant && threshold == o.threshold && tileShift == o.tileShift && mode == o.mode;
+    }
 };
+
+// A candidate replaces the best so far only when it is this much faster
+// (launch-to-launch spread of one candidate is about 1%).
+constexpr float kMinGain = 0.02f;
 
 // Milliseconds per sample of `samples` samples from a reset accumulation,
 // between events on the context's stream (the reset is outside the window).
@@ -69,25 +79,30 @@ extern "C" int rc_autotune(rc_ctx *ctx, const rc_camera *cam, const rc_pt_params
         return hip_fail(ctx, e, "hipEventCreate");
     }
     int rc = RC_OK;
-    Candidate best = keep;
+    const Candidate start{keep.variant, keep.threshold, keep.tileShift, RC_PT_LANES};
+    Candidate best = start;
     float bestMs = FLT_MAX;
     auto trial = [&](const Candidate &c) {
+        if (!(bestMs == FLT_MAX) && c == best) return;  // already timed
         float ms = 0.0f;
         if (rc == RC_OK) rc = time_candidate(ctx, *cam, *p, 1, c, a, b, ms);  // first launch of a variant: warm-up
         if (rc == RC_OK) rc = time_candidate(ctx, *cam, *p, samplesPerTrial, c, a, b, ms);
-        if (rc == RC_OK && ms < bestMs) {
+        if (rc == RC_OK && (bestMs == FLT_MAX || ms < bestMs * (1.0f - kMinGain))) {
             bestMs = ms;
             best = c;
         }
     };
-    // coordinate descent: kernel variant, then wait threshold, then tile shape;
-    // last, the megakernel, which wins when every lane has a single short unit
-    // (e.g. one-sample passes over a small scene)
-    for (int v = 0; v < RC_LANE_VARIANTS; ++v) trial(Candidate{v, keep.threshold, keep.tileShift, RC_PT_LANES});
-    const int variant = best.variant;
-    for (int th : {16, 24, 32, 40, 48, 56}) trial(Candidate{variant, th, keep.tileShift, RC_PT_LANES});
-    const int threshold = best.threshold;
-    for (int sh = 2; sh <= 5; ++sh) trial(Candidate{variant, threshold, sh, RC_PT_LANES});
+    // an untimed run at the trial size first: it grows the sample buffer and
+    // warms the caches, which would otherwise be charged to the first candidate
+    float warm = 0.0f;
+    rc = time_candidate(ctx, *cam, *p, samplesPerTrial, start, a, b, warm);
+    // coordinate descent from the current tuning: kernel variant, then wait
+    // threshold, then tile shape; last, the megakernel, which wins when every
+    // lane has a single short unit (e.g. one-sample passes over a small scene)
+    trial(start);
+    for (int v = 0; v < RC_LANE_VARIANTS; ++v) trial(Candidate{v, best.threshold, best.tileShift, RC_PT_LANES});
+    for (int th : {16, 24, 32, 40, 48, 56}) trial(Candidate{best.variant, th, best.tileShift, RC_PT_LANES});
+    for (int sh = 2; sh <= 5; ++sh) trial(Candidate{best.variant, best.threshold, sh, RC_PT_LANES});
     trial(Candidate{best.variant, best.threshold, best.tileShift, RC_PT_MEGAKERNEL});
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
