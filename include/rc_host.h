@@ -20,6 +20,7 @@
 #ifndef RC_HOST_H
 #define RC_HOST_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -76,6 +77,48 @@ void rcs_lookat_basis(const float *eye, const float *target,
                       float *fwd, float *right, float *up);
 /* (float)tan(atan(0.012 / (focalLengthMm / 1000))) */
 float rcs_tan_half_fov(float focalLengthMm);
+
+/* ---- scene files (rc_scenejson.cpp) --------------------------------------
+ * SceneSerializer.load (src/main/java/com/retrocam/scene/SceneSerializer.java:72-212)
+ * with the Java reader's scanning rules, so a file yields the values the Java
+ * host reads from it, and SceneEditor.buildScene (SceneEditor.java:101-107).
+ * The writer (SceneSerializer.save) is retrocam_amd/scene_file.py. */
+
+enum { RCS_OBJ_BOX = 0, RCS_OBJ_SPHERE = 1 };  /* SceneObject.Type */
+enum {
+    RCS_JSON_OK = 0,
+    RCS_JSON_ERR_ARG = 1,
+    RCS_JSON_ERR_OOM = 2,
+    RCS_JSON_ERR_STRING = 3  /* a string scan would run past the text (the Java reader throws) */
+};
+
+/* SceneObject fields (SceneObject.java:14-30): position, scale (a sphere's
+ * radius is scale[0]), XYZ rotation in degrees about the position. */
+typedef struct rcs_json_object {
+    int32_t type;
+    float pos[3];
+    float scale[3];
+    float rot[3];
+    int32_t stacks, slices;    /* 24 unless the file gives > 2 */
+    int32_t materialIndex;     /* >= 0 */
+} rcs_json_object;
+
+typedef struct rcs_doc rcs_doc;
+
+/* Parses len bytes of scene-file text; NULL on failure with *status (may be NULL) set. */
+rcs_doc *rcs_json_parse(const char *text, size_t len, int32_t *status);
+void rcs_json_free(rcs_doc *d);
+int32_t rcs_json_material_count(const rcs_doc *d);
+int32_t rcs_json_object_count(const rcs_doc *d);
+/* Material i in the Material.write layout (16 floats, albedoTex -1) and its name
+ * (NUL-terminated, truncated to nameCap - 1 bytes). Returns the name's length, or -1. */
+int32_t rcs_json_get_material(const rcs_doc *d, int32_t i, float *mat16, char *name, int32_t nameCap);
+int32_t rcs_json_get_object(const rcs_doc *d, int32_t i, rcs_json_object *o, char *name, int32_t nameCap);
+/* SceneObject.toPrimitive (SceneObject.java:80-88): the box or sphere, rotated
+ * when any angle is non-zero. Returns triangles added, or -1. */
+int32_t rcs_add_object(rcs_scene *s, const rcs_json_object *o);
+/* The document's materials, then its objects, in file order. Returns triangles added, or -1. */
+int32_t rcs_json_build(const rcs_doc *d, rcs_scene *s);
 
 /* ---- per-video-frame inputs (rc_temporal.cpp) ---------------------------
  * The values the render path takes per video frame (rc_temporal, the camera's

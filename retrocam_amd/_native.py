@@ -122,6 +122,13 @@ class RctState(C.Structure):
 
 
 RCK_STEP, RCK_LINEAR, RCK_BEZIER = 0, 1, 2
+RCS_OBJ_BOX, RCS_OBJ_SPHERE = 0, 1
+RCS_JSON_OK, RCS_JSON_ERR_ARG, RCS_JSON_ERR_OOM, RCS_JSON_ERR_STRING = 0, 1, 2, 3
+
+
+class RcsJsonObject(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pos", _F * 3), ("scale", _F * 3), ("rot", _F * 3),
+                ("stacks", C.c_int32), ("slices", C.c_int32), ("materialIndex", C.c_int32)]
 
 _FP = C.POINTER(C.c_float)
 _VP = C.c_void_p
@@ -214,6 +221,14 @@ HOST_SYMBOLS = {
     "rcj_frame_dt": (C.c_float, [C.c_float]),
     "rcj_total_frames": (C.c_int32, [C.c_float, C.c_float]),
     "rcj_canonical_time": (C.c_float, [C.c_int32, C.c_float]),
+    "rcs_json_parse": (C.c_void_p, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int32)]),
+    "rcs_json_free": (None, [_VP]),
+    "rcs_json_material_count": (C.c_int32, [_VP]),
+    "rcs_json_object_count": (C.c_int32, [_VP]),
+    "rcs_json_get_material": (C.c_int32, [_VP, C.c_int32, _FP, C.c_char_p, C.c_int32]),
+    "rcs_json_get_object": (C.c_int32, [_VP, C.c_int32, C.POINTER(RcsJsonObject), C.c_char_p, C.c_int32]),
+    "rcs_add_object": (C.c_int32, [_VP, C.POINTER(RcsJsonObject)]),
+    "rcs_json_build": (C.c_int32, [_VP, _VP]),
     "rck_new": (C.c_void_p, []),
     "rck_free": (None, [_VP]),
     "rck_add": (C.c_int32, [_VP, C.c_float, C.c_float, C.c_int32]),

@@ -66,7 +66,8 @@ def build_host() -> str:
     tmp = out + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
           f"-I{INC}", os.path.join(CSRC, "host", "rc_host.cpp"),
-          os.path.join(CSRC, "host", "rc_temporal.cpp"), "-o", tmp])
+          os.path.join(CSRC, "host", "rc_temporal.cpp"), os.path.join(CSRC, "host", "rc_scenejson.cpp"),
+          "-o", tmp])
     os.replace(tmp, out)
     return out
 

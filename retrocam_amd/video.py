@@ -17,7 +17,9 @@ replayed with each frame's camera (rc_frame_set_camera) and temporal inputs
 sizes every buffer the recording uses), frame 1 is recorded and replayed.
 Keyframes on post settings or scene objects change what the frame computes
 beyond the camera and the temporal inputs; with such tracks every frame runs
-eagerly.
+eagerly, and with scene-object tracks the scene editor's state is rebuilt and
+uploaded per frame (RenderPipeline.applyKeyframes re-uploads the scene when
+KeyframeTimeline.hasSceneObjectTracks, KeyframeTimeline.java:88-96).
 """
 from __future__ import annotations
 
@@ -28,7 +30,7 @@ import numpy as np
 
 from . import _native as N
 from .post import PostProcessStack, TemporalInputs
-from .renderer import RenderContext, RenderError, Renderer
+from .renderer import RenderContext, RenderError, Renderer, SceneUploader
 from .scene import CameraSpec, PtParams
 from .temporal import (KeyframeTimeline, LensSettings, OrbitRig, PostSettings, TemporalSettings, TemporalState,
                        canonical_time, frame_dt, total_frames)
@@ -88,6 +90,7 @@ class VideoScene:
     temporal: TemporalSettings = field(default_factory=TemporalSettings)
     timeline: KeyframeTimeline | None = None
     camera_kw: dict = field(default_factory=dict)  # further CameraSpec fields (blades, lcaDelta, pinhole)
+    editor: object = None  # SceneEditor whose objects scene-object tracks animate
 
 
 class VideoRenderer:
@@ -130,10 +133,10 @@ class VideoRenderer:
         dt = job.frame_dt()
         timeline = scene.timeline
         post_animated = timeline is not None and timeline.has_tracks_on(scene.post)
-        graph_ok = self.use_graph and not post_animated
-        if timeline is not None and timeline.has_scene_object_tracks():
-            raise NotImplementedError("scene-object keyframes need a scene re-upload per frame (SceneEditor), "
-                                      "which is not part of this library")
+        objects_animated = timeline is not None and timeline.has_scene_object_tracks()
+        if objects_animated and scene.editor is None:
+            raise ValueError("scene-object keyframes need VideoScene.editor (the SceneEditor they animate)")
+        graph_ok = self.use_graph and not post_animated and not objects_animated
         if job.output_path is not None:
             os.makedirs(os.path.dirname(job.frame_path(0)) or ".", exist_ok=True)
         temporal = TemporalState()
@@ -147,6 +150,8 @@ class VideoRenderer:
                     timeline.apply(t)
                     if post_animated:
                         scene.post.write_to(self.post_params)
+                    if objects_animated:
+                        SceneUploader(self.ctx).upload(scene.editor.build())
                 temporal.set_focal_dist_target(scene.lens.focusDistM)
                 temporal.update_for_video_frame(t, dt, scene.temporal)
                 cam = scene.rig.camera(job.width, job.height, scene.lens, temporal.focalDist, **scene.camera_kw)

@@ -152,3 +152,35 @@ def test_keyframed_clip_replayed_equals_eager(gpu_ctx, cornell, tmp_path):
     for g in graph:
         with open(g.path, "rb") as f:
             assert f.read(8) == b"\x89PNG\r\n\x1a\n"
+
+
+def test_scene_object_keyframes_rebuild_the_scene(gpu_ctx):
+    # RenderPipeline re-uploads the scene per frame when a track moves a scene
+    # object; each frame must equal an eager render of the editor at that time.
+    from retrocam_amd.scene import PtParams
+    from retrocam_amd.scene_file import SceneEditor
+    ed = SceneEditor.create_default()
+    block = ed.objects[7]  # Metal Block
+    tl = T.KeyframeTimeline(0.25)
+    tr = tl.add_track(block, "pos.x", T.TargetType.SCENE_OBJECT)
+    tr.add_keyframe(0.0, 1.5)
+    tr.add_keyframe(0.25, -0.5)
+    scene = VideoScene(editor=ed, timeline=tl, camera_kw={"pinhole": True})
+    job = VideoJob(64, 36, fps=12.0, duration_seconds=0.25, samples_per_frame=2)
+    r = Renderer(gpu_ctx)
+    r.set_pt_mode(N.RC_PT_LANES)
+    post = PostProcessStack(gpu_ctx)
+    pp = make_params(enable=VHS_PASSES, atrous=True, iterations=2)
+    params = PtParams(1, 1.0, 6)
+    recs = VideoRenderer(gpu_ctx, r, post, params, pp).render(job, scene, keep_images=True)
+    assert len(recs) == 3 and not any(g.replayed for g in recs)
+    assert not np.array_equal(recs[0].image, recs[2].image)
+    for rec in (recs[0], recs[2]):
+        ref_ed = SceneEditor.create_default()
+        ref_ed.objects[7].px = tr.evaluate(rec.time, 1.5)
+        SceneUploader(gpu_ctx).upload(ref_ed.build())
+        cam = T.OrbitRig().camera(64, 36, T.LensSettings(), rec.focal_dist, pinhole=True)
+        r.reset()
+        r.render(cam, params, n_samples=2)
+        post.run_on_accum(pp, TemporalInputs(rec.time, rec.agc_gain, rec.index), read=False)
+        assert np.array_equal(post.export_frame(), rec.image), rec.index
