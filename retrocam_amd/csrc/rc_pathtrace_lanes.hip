@@ -538,8 +538,9 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
 //   7: W 4, NS 20, TRI (default)                      8: W 3, NS 26, TRI
 //   9: W 4, NS 20, dual     10: W 5, NS 16, dual
 // On the 250k-triangle frame at 80 samples: variant 0 177 ms, 4 141 ms, 6 138 ms,
-// 7 121 ms (126 VGPRs, no register spills; the 5-wave and higher variants spill
-// around the shading block and move 5x the HBM bytes in scratch traffic), 8 144 ms.
+// 7 121 ms (no register spills; 117 VGPRs since the camera is read through the
+// constant address space; the 5-wave and higher variants spill around the shading
+// block and move about 5x the HBM bytes in scratch traffic), 8 144 ms.
 template <bool STATS, int W, int NS, int WALK>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
 pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh, float4 *cont) {

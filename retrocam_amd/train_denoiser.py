@@ -70,6 +70,8 @@ def main(argv=None):
     ap.add_argument("--frame-seeds", type=int, default=64,
                     help="full-frame scenes (seeds 2..) cut into crops; seed 1 is the benchmark frame, never used")
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
+    ap.add_argument("--spp-in", default="4,8,16,32,64,128,256",
+                    help="input sample counts the levels are drawn from (comma-separated)")
     args = ap.parse_args(argv)
 
     def log(*a):
@@ -77,11 +79,14 @@ def main(argv=None):
 
     ctx = RenderContext(0)
     t0 = time.perf_counter()
-    raw, tgt = make_dataset(ctx, range(args.scenes), size=args.size, spp_ref=args.ref_spp, log=log,
+    spp_in = tuple(int(v) for v in args.spp_in.split(",") if v.strip())
+    if len(spp_in) < args.levels:
+        raise SystemExit("--spp-in needs at least --levels values")
+    raw, tgt = make_dataset(ctx, range(args.scenes), size=args.size, spp_in=spp_in, spp_ref=args.ref_spp, log=log,
                             levels=args.levels, seed=args.seed)
     if args.frame_seeds > 0:
-        fr, ft = make_frame_crops(ctx, range(2, 2 + args.frame_seeds), size=args.size, spp_ref=args.ref_spp,
-                                  levels=args.levels, log=log, seed=args.seed + 1)
+        fr, ft = make_frame_crops(ctx, range(2, 2 + args.frame_seeds), size=args.size, spp_in=spp_in,
+                                  spp_ref=args.ref_spp, levels=args.levels, log=log, seed=args.seed + 1)
         raw, tgt = np.concatenate([raw, fr]), np.concatenate([tgt, ft])
         del fr, ft
     t_data = time.perf_counter() - t0
@@ -107,7 +112,8 @@ def main(argv=None):
     ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.eval_ref_spp) \
         if args.eval_scenes > 0 else np.full((1, 3), np.nan)
     summary = {
-        "scenes": args.scenes, "levels": args.levels, "frame_seeds": args.frame_seeds, "pairs": int(raw.shape[0]),
+        "scenes": args.scenes, "levels": args.levels, "spp_in": list(spp_in), "frame_seeds": args.frame_seeds,
+        "pairs": int(raw.shape[0]),
         "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
         "batch": args.batch, "lr": args.lr, "params": net.n,
         "dataset_s": round(t_data, 2), "train_s": round(t_train, 2),
