@@ -345,7 +345,7 @@ def main():
                     help="time frames at this spp (with --denoiser) instead of the U-Net's operating point")
     ap.add_argument("--denoiser", choices=["atrous", "net"], default="atrous", help="denoiser of --fixed-spp frames")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
-    ap.add_argument("--sweep", default="4,6,8,10,12,14,16,20,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
+    ap.add_argument("--sweep", default="4,5,6,7,8,10,12,14,16,20,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
                     help="spp values searched for the relMSE <= 0.01 operating points")
     ap.add_argument("--cpu-spp", type=int, default=32, help="samples timed for the CPU baseline (0 = skip)")
     ap.add_argument("--pt-mode", choices=["lanes", "wavefront", "megakernel"], default="lanes")

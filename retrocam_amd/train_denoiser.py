@@ -70,6 +70,10 @@ def main(argv=None):
     ap.add_argument("--frame-seeds", type=int, default=64,
                     help="full-frame scenes (seeds 2..) cut into crops; seed 1 is the benchmark frame, never used")
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
+    ap.add_argument("--init", default=None, help="start from these weights (.npy) instead of a seeded init")
+    ap.add_argument("--scene-offset", type=int, default=0, help="training views use seeds offset .. offset+scenes-1")
+    ap.add_argument("--frame-offset", type=int, default=2,
+                    help="full-frame seeds offset .. offset+frame-seeds-1 (seed 1, the benchmark frame, is refused)")
     ap.add_argument("--spp-in", default="4,8,16,32,64,128,256",
                     help="input sample counts the levels are drawn from (comma-separated)")
     args = ap.parse_args(argv)
@@ -82,16 +86,22 @@ def main(argv=None):
     spp_in = tuple(int(v) for v in args.spp_in.split(",") if v.strip())
     if len(spp_in) < args.levels:
         raise SystemExit("--spp-in needs at least --levels values")
-    raw, tgt = make_dataset(ctx, range(args.scenes), size=args.size, spp_in=spp_in, spp_ref=args.ref_spp, log=log,
+    if args.frame_offset <= 1 < args.frame_offset + args.frame_seeds:
+        raise SystemExit("full-frame seed 1 is the benchmark frame: use --frame-offset >= 2")
+    scene_seeds = range(args.scene_offset, args.scene_offset + args.scenes)
+    if any(s >= 1_000_000 for s in (scene_seeds[-1:] if args.scenes else [])):
+        raise SystemExit("training view seeds >= 1000000 are the held-out evaluation views")
+    raw, tgt = make_dataset(ctx, scene_seeds, size=args.size, spp_in=spp_in, spp_ref=args.ref_spp, log=log,
                             levels=args.levels, seed=args.seed)
     if args.frame_seeds > 0:
-        fr, ft = make_frame_crops(ctx, range(2, 2 + args.frame_seeds), size=args.size, spp_in=spp_in,
+        fr, ft = make_frame_crops(ctx, range(args.frame_offset, args.frame_offset + args.frame_seeds), size=args.size,
+                                  spp_in=spp_in,
                                   spp_ref=args.ref_spp, levels=args.levels, log=log, seed=args.seed + 1)
         raw, tgt = np.concatenate([raw, fr]), np.concatenate([tgt, ft])
         del fr, ft
     t_data = time.perf_counter() - t0
     log(f"[train] dataset: {raw.shape[0]} views {args.size}x{args.size} in {t_data:.1f} s")
-    net = NetDenoiser(ctx).init(args.seed)
+    net = NetDenoiser(ctx).load(args.init) if args.init else NetDenoiser(ctx).init(args.seed)
     net.set_dataset(raw, tgt)
     t1 = time.perf_counter()
     chunk = 250
@@ -112,6 +122,7 @@ def main(argv=None):
     ev = evaluate(ctx, net, range(1_000_000, 1_000_000 + args.eval_scenes), args.size, 8, args.eval_ref_spp) \
         if args.eval_scenes > 0 else np.full((1, 3), np.nan)
     summary = {
+        "init": args.init, "scene_offset": args.scene_offset, "frame_offset": args.frame_offset,
         "scenes": args.scenes, "levels": args.levels, "spp_in": list(spp_in), "frame_seeds": args.frame_seeds,
         "pairs": int(raw.shape[0]),
         "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
