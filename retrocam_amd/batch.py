@@ -66,3 +66,54 @@ class ViewBatch:
         self.ctx.check(self._lib.rc_views_read(self.ctx.h, int(view), int(which), out.ctypes.data_as(C.c_void_p),
                                                nbytes))
         return out
+
+
+# Units (one sample of one pixel) a launch needs before every lane of the GPU has
+# work for the whole launch: 256 CUs x 4 SIMDs x 4 waves x 64 lanes, 32 units each.
+FILL_UNITS = 256 * 4 * 4 * 64 * 32
+MAX_VIEWS = 4096  # views per rc_views_render
+
+
+def request_units(q: ViewRequest) -> int:
+    """The work of a request in units: pixels x samples."""
+    return int(q.camera.width) * int(q.camera.height) * int(q.spp)
+
+
+def plan_requests(reqs, fill_units: int = FILL_UNITS, max_views: int = MAX_VIEWS) -> list:
+    """Launch groups for serving requests that all arrive together: shortest job
+    first (ascending units, ties in arrival order), consecutive requests packed
+    into one launch while the group stays at or below fill_units, larger
+    requests alone. Returns a list of groups, each a list of request indices.
+
+    Shortest-first minimises the mean (and median) latency of a set of jobs on
+    one machine; packing only requests too small to fill the GPU on their own
+    keeps the throughput of one launch per request for the rest."""
+    order = sorted(range(len(reqs)), key=lambda i: (request_units(reqs[i]), i))
+    groups, cur, cur_units = [], [], 0
+    for i in order:
+        u = request_units(reqs[i])
+        if cur and (cur_units + u > fill_units or len(cur) >= max_views):
+            groups.append(cur)
+            cur, cur_units = [], 0
+        cur.append(i)
+        cur_units += u
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def serve(vb: ViewBatch, reqs, post_params, temporal, on_done=None, fill_units: int = FILL_UNITS) -> list:
+    """Serves requests in plan_requests order: per group one rc_views_render, then
+    each view's post chain (rc_views_post). on_done(request_index, view) runs
+    after a request's chain is queued, while its images are still readable
+    through `vb` (the next group re-lays out the views). Returns the order in
+    which requests completed."""
+    done = []
+    for group in plan_requests(reqs, fill_units):
+        vb.render([reqs[i] for i in group], reset=True)
+        for v, i in enumerate(group):
+            vb.post(v, post_params, temporal)
+            if on_done is not None:
+                on_done(i, v)
+            done.append(i)
+    return done

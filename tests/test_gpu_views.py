@@ -7,7 +7,7 @@ import pytest
 
 from retrocam_amd import _native as N
 from retrocam_amd import workloads
-from retrocam_amd.batch import ViewBatch, ViewRequest
+from retrocam_amd.batch import ViewBatch, ViewRequest, plan_requests, request_units, serve
 from retrocam_amd.post import VHS_PASSES, PostProcessStack, TemporalInputs, make_params
 from retrocam_amd.renderer import RenderContext, RenderError, Renderer, SceneUploader
 from retrocam_amd.scene import CameraSpec
@@ -98,3 +98,30 @@ def test_views_reject_bad_requests(gpu_ctx):
         vb.render([ViewRequest(N.RC_MAX_SCENE_SLOTS - 1, reqs[0].camera, reqs[0].params, 1)])  # empty slot
     with pytest.raises(RenderError):
         vb.render([ViewRequest(0, reqs[0].camera, reqs[0].params, 0)])
+
+
+def test_scheduled_serving_equals_single_renders(gpu_ctx):
+    # serve() packs the small requests into shared launches and runs the rest
+    # alone, shortest first: every request's display must still equal its own
+    # single-view render.
+    scenes, reqs = requests()
+    reqs = reqs + [ViewRequest(1, reqs[1].camera, reqs[1].params, 9)]
+    Renderer(gpu_ctx).set_pt_mode(N.RC_PT_LANES)
+    vb = ViewBatch(gpu_ctx)
+    for k, sc in enumerate(scenes):
+        vb.upload_scene(k, sc)
+    pp = make_params(enable=VHS_PASSES, atrous=True, iterations=3)
+    fill = request_units(reqs[0]) + request_units(reqs[2])  # forces several launches
+    groups = plan_requests(reqs, fill_units=fill)
+    assert 1 < len(groups) < len(reqs)
+    got = {}
+
+    def keep(i, v):
+        got[i] = (vb.read(v, N.RC_BUF_ACCUM), vb.read(v, N.RC_BUF_DISPLAY))
+
+    order = serve(vb, reqs, pp, TemporalInputs(0.5, 1.0, 3), on_done=keep, fill_units=fill)
+    assert order == [i for g in groups for i in g]
+    for i, q in enumerate(reqs):
+        ref = single(scenes[q.scene], q, [q.spp])
+        assert np.array_equal(got[i][0], ref[N.RC_BUF_ACCUM]), i
+        assert np.array_equal(got[i][1], ref["rgba8"]), i
