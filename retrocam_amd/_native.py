@@ -184,6 +184,7 @@ HIP_SYMBOLS = {
     "rc_frame_replay": (C.c_int, [_VP, C.POINTER(RcTemporal)]),
     "rc_frame_release": (C.c_int, [_VP]),
     "rc_frame_set_camera": (C.c_int, [_VP, C.POINTER(RcCamera)]),
+    "rc_set_net_flips": (C.c_int, [_VP, C.c_int32]),
     "rc_get_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_set_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_autotune": (C.c_int, [_VP, C.POINTER(RcCamera), C.POINTER(RcPtParams), C.c_int32, C.POINTER(RcTuning),

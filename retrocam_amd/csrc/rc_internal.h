@@ -151,6 +151,7 @@ struct rc_ctx {
     int32_t laneTileShift = 3;  // unit tiles of 8 x 8 pixels
     rc::Net *net = nullptr;
     int32_t denoiser = RC_DENOISE_ATROUS;
+    int32_t netFlips = 1;  // network passes per denoise (rc_set_net_flips)
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
     int countersEnabled = 0;
     // captured frame (rc_frame_*): the graph, its instance, and the temporal

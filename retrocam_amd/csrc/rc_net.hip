@@ -187,6 +187,95 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
     }
 }
 
+// The same convolution with the input staged in LDS: a block computes an 8 x 16
+// tile of output pixels of one image (wave w: rows 2w, 2w+1, 16 pixels each),
+// after loading the tile's 10 x 18 input halo, every input channel, once (zero
+// outside the image; the a source upsampled in place). The K order per output
+// (tap, then 32-channel chunk) and the operands are those of conv3x3_fwd, so
+// the outputs are bit-identical; every input pixel is read from HBM/L2 once
+// per tile instead of once per tap. LDS rows are padded by 8 channels (16 B)
+// against bank conflicts of the 16 pixel lanes.
+constexpr int kLdsTY = 8, kLdsTX = 16, kLdsHY = kLdsTY + 2, kLdsHX = kLdsTX + 2;
+
+__host__ __device__ constexpr size_t conv_lds_bytes(int cin) {
+    return (size_t)kLdsHY * kLdsHX * (size_t)(cin + 8) * 2;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) conv3x3_fwd_lds(ConvArgs A) {
+    extern __shared__ __align__(16) uint16_t halo[];
+    const int cin = A.ca + A.cb, rs = cin + 8;
+    const int tilesX = (A.W + kLdsTX - 1) / kLdsTX, tilesY = (A.H + kLdsTY - 1) / kLdsTY;
+    int bi = blockIdx.x;
+    const int tx0 = (bi % tilesX) * kLdsTX;
+    bi /= tilesX;
+    const int ty0 = (bi % tilesY) * kLdsTY;
+    const int n = bi / tilesY;
+    const int Hh = A.H >> 1, Wh = A.W >> 1;
+    const int per = cin >> 3;  // 16-byte items per halo pixel
+    const int items = kLdsHY * kLdsHX * per;
+    for (int it = threadIdx.x; it < items; it += 256) {
+        const int hp = it / per, c = (it - hp * per) * 8;
+        const int hy = hp / kLdsHX, hx = hp - hy * kLdsHX;
+        const int ys = ty0 + hy - 1, xs = tx0 + hx - 1;
+        u32x4 v = u32x4{0u, 0u, 0u, 0u};
+        if (ys >= 0 && ys < A.H && xs >= 0 && xs < A.W) {
+            const uint16_t *src;
+            if (c < A.ca) {
+                const size_t pa = A.aUp ? ((size_t)n * Hh + (ys >> 1)) * Wh + (xs >> 1)
+                                        : ((size_t)n * A.H + ys) * A.W + xs;
+                src = A.a + pa * A.ca + c;
+            } else {
+                src = A.b + (((size_t)n * A.H + ys) * A.W + xs) * A.cb + (c - A.ca);
+            }
+            v = *(const u32x4 *)src;
+        }
+        *(u32x4 *)(halo + (size_t)hp * rs + c) = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kq = 8 * (lane >> 4);
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3, dx = tap % 3;  // halo offsets of the tap
+        const uint16_t *r0 = halo + ((size_t)(2 * wave + dy) * kLdsHX + (lane & 15) + dx) * rs + kq;
+        const uint16_t *r1 = r0 + (size_t)kLdsHX * rs;
+        const uint16_t *wt = A.w + ((size_t)(lane & 15) * 9 + tap) * cin + kq;
+        for (int c0 = 0; c0 < cin; c0 += 32) {
+            const bf16x8 a0 = load8(r0 + c0), a1 = load8(r1 + c0);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 bv = load8(wt + (size_t)t * 16 * 9 * cin + c0);
+                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[0][t], 0, 0, 0);
+                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[1][t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + (lane & 15);
+        const float bb = A.bias ? A.bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const int y = ty0 + 2 * wave + m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = tx0 + 4 * (lane >> 4) + r;
+                if (x >= A.W || y >= A.H) continue;
+                const size_t p = ((size_t)n * A.H + y) * A.W + x;
+                float v = acc[m][t][r] + bb;
+                if (A.relu) v = fmaxf(v, 0.0f);
+                if (A.outB) A.outB[p * A.cout + co] = f2bf(v);
+                else A.outF[p * A.cout + co] = v;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution. Block (chunk, tap): pixels
 // [chunk*pixPerChunk, +pixPerChunk) in steps of 32 (the MFMA K); dZ and the
@@ -394,6 +483,52 @@ struct HeadArgs {
     uint16_t *dz;          // [npix][32]
     float invCount;
 };
+// One output channel of the 1x1 head at pixel p: y = bias + W d1 + the
+// residual log-radiance feature, x = m * expm1(min(y, 20)) with the albedo
+// demodulation factor m (the denoised radiance before the clamp at 0).
+__device__ __forceinline__ float head_channel(const uint16_t *wb, const float *bias, const uint16_t *feat, size_t p,
+                                              int j, const float *d1v, float &y, float &m) {
+    float o = bias[j];
+    for (int c = 0; c < 32; ++c) o += bf2f(wb[j * 32 + c]) * d1v[c];
+    y = o + bf2f(feat[p * 32 + j]);
+    const float yc = fminf(y, 20.0f);
+    m = fmaxf(bf2f(feat[p * 32 + 3 + j]), 0.01f);
+    return m * expm1f(yc);
+}
+
+// Head of a flip ensemble: the batch holds `passes` images of (Hp, Wp), image k
+// the frame mirrored by k (bit 0 left-right, bit 1 top-bottom). Pixel (x, y) of
+// the w x h output is ((v0 + v1) + v2 + v3) * (1 / passes) of the clamped head
+// outputs vk at the mirrored positions; one pass writes v0 itself.
+struct EnsembleArgs {
+    const uint16_t *d1, *feat, *wb;
+    const float *bias;
+    int Wp, Hp, w, h, passes;
+    float4 *dst;
+};
+__global__ void __launch_bounds__(256) head_ensemble_kernel(EnsembleArgs A) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.w * A.h) return;
+    const int x = i % A.w, y = i / A.w;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int k = 0; k < A.passes; ++k) {
+        const int xk = (k & 1) ? A.w - 1 - x : x, yk = (k & 2) ? A.h - 1 - y : y;
+        const size_t p = (size_t)k * A.Wp * A.Hp + (size_t)yk * A.Wp + xk;
+        float d1v[32];
+#pragma unroll
+        for (int c = 0; c < 32; ++c) d1v[c] = bf2f(A.d1[p * 32 + c]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float yy, m;
+            const float v = fmaxf(head_channel(A.wb, A.bias, A.feat, p, j, d1v, yy, m), 0.0f);
+            acc[j] = k == 0 ? v : acc[j] + v;
+        }
+    }
+    const float s = 1.0f / (float)A.passes;
+    A.dst[i] = make_float4(acc[0] * s, acc[1] * s, acc[2] * s, 1.0f);
+}
+
 __global__ void __launch_bounds__(256) head_kernel(HeadArgs A) {
     __shared__ float red[256];
     const int p = blockIdx.x * 256 + threadIdx.x;
@@ -407,12 +542,8 @@ __global__ void __launch_bounds__(256) head_kernel(HeadArgs A) {
         float xo[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            float o = A.bias[j];
-            for (int c = 0; c < 32; ++c) o += bf2f(A.wb[j * 32 + c]) * d1v[c];
-            float y = o + bf2f(A.feat[(size_t)p * 32 + j]);
-            float yc = fminf(y, 20.0f);
-            float m = fmaxf(bf2f(A.feat[(size_t)p * 32 + 3 + j]), 0.01f);
-            xo[j] = m * expm1f(yc);
+            float y, m;
+            xo[j] = head_channel(A.wb, A.bias, A.feat, p, j, d1v, y, m);
             if (A.target) {
                 float r = A.target[(size_t)p * 3 + j];
                 float den = r * r + 0.01f;
@@ -528,12 +659,17 @@ __global__ void raw_features(const float *raw, size_t npix, uint16_t *feat) {
 }
 // Features of a rendered frame, padded to (Hp, Wp) by edge replication:
 // radiance from the post image (p00 output), albedo/normal/depth from the
-// RGBA16F G-buffers, mean luma^2 = variance sum / spp.
+// RGBA16F G-buffers, mean luma^2 = variance sum / spp. Image k of the
+// `passes` images written is the frame mirrored by k (bit 0 left-right, bit 1
+// top-bottom) before the padding.
 __global__ void frame_features(const float4 *hdr, const ushort4 *gbuf, const ushort4 *galb, const float *variance,
-                               int total, int w, int h, int Wp, int Hp, uint16_t *feat) {
+                               int total, int w, int h, int Wp, int Hp, int passes, uint16_t *feat) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Wp * Hp) return;
-    const int x = min(i % Wp, w - 1), y = min(i / Wp, h - 1);
+    if (i >= passes * Wp * Hp) return;
+    const int k = i / (Wp * Hp), j = i - k * (Wp * Hp);
+    int x = min(j % Wp, w - 1), y = min(j / Wp, h - 1);
+    if (k & 1) x = w - 1 - x;
+    if (k & 2) y = h - 1 - y;
     const size_t s = (size_t)y * w + x;
     float4 c = hdr[s];
     ushort4 g = gbuf[s], a = galb[s];
@@ -727,6 +863,19 @@ void launch_conv_nt(hipStream_t s, const ConvArgs &A) {
     hipLaunchKernelGGL((conv3x3_fwd<NT, 2>), dim3(grid), dim3(256), 0, s, A);
 }
 
+// conv3x3_fwd_lds for 32 or 64 output channels and at most 128 input channels
+// (LDS: 49 KB per block at 128); false when it does not apply.
+bool launch_conv_lds(hipStream_t s, const ConvArgs &A) {
+    const int cin = A.ca + A.cb;
+    if ((A.cout != 32 && A.cout != 64) || cin > 128 || (cin & 31) || (A.ca & 7) || (A.cb & 7)) return false;
+    const unsigned tiles = (unsigned)A.N * (unsigned)((A.H + kLdsTY - 1) / kLdsTY) *
+                           (unsigned)((A.W + kLdsTX - 1) / kLdsTX);
+    const size_t lds = conv_lds_bytes(cin);
+    if (A.cout == 32) hipLaunchKernelGGL(conv3x3_fwd_lds<2>, dim3(tiles), dim3(256), lds, s, A);
+    else hipLaunchKernelGGL(conv3x3_fwd_lds<4>, dim3(tiles), dim3(256), lds, s, A);
+    return true;
+}
+
 hipError_t launch_conv(hipStream_t s, const ConvArgs &A) {
     switch (A.cout / 16) {
         case 2: launch_conv_nt<2>(s, A); break;
@@ -771,6 +920,8 @@ hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W) {
         A.cout = sp.cout;
         A.relu = 1;
         A.outB = n.act[L];
+        // the three finest levels stage their input tiles in LDS
+        if (sp.level <= 2 && launch_conv_lds(s, A)) continue;
         hipError_t e = launch_conv(s, A);
         if (e != hipSuccess) return e;
     }
@@ -970,28 +1121,34 @@ void free_net(Net *n) {
 bool net_ready(const rc_ctx *ctx) { return ctx->net && ctx->net->ready; }
 
 // Denoises the post image src (w x h RGBA32F, p00 output) into dst using the
-// frame's G-buffers.
+// frame's G-buffers: the mean of ctx->netFlips network passes over the frame
+// as is, mirrored left-right, top-bottom and both (in that order), each pass's
+// output mirrored back; ((p0 + p1) + p2 + p3) * (1 / passes) per channel. The
+// passes run as one batch of images through the network.
 int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo,
                     const float *variance, int total, float4 *dst, int w, int h) {
     Net &n = *ctx->net;
     const int Wp = (w + 15) / 16 * 16, Hp = (h + 15) / 16 * 16;
-    hipError_t e = ensure_ws(ctx, n, 1, Hp, Wp, false);
+    const int passes = ctx->netFlips;
+    if ((size_t)passes * Wp * Hp > (size_t)64 << 20)
+        return set_error(ctx, RC_ERR_LIMIT, "net denoise: passes x padded frame above 64 Mpixels (lower rc_set_net_flips)");
+    hipError_t e = ensure_ws(ctx, n, passes, Hp, Wp, false);
     if (e != hipSuccess) return hip_fail(ctx, e, "net workspace");
-    hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)Wp * Hp)), dim3(256), 0, ctx->stream, src, gbuf,
-                       galbedo, variance, total, w, h, Wp, Hp, n.feat);
-    if ((e = forward(ctx, n, 1, Hp, Wp)) != hipSuccess) return hip_fail(ctx, e, "net forward");
-    HeadArgs hd{};
+    hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)passes * Wp * Hp)), dim3(256), 0, ctx->stream, src,
+                       gbuf, galbedo, variance, total, w, h, Wp, Hp, passes, n.feat);
+    if ((e = forward(ctx, n, passes, Hp, Wp)) != hipSuccess) return hip_fail(ctx, e, "net forward");
+    EnsembleArgs hd{};
     hd.d1 = n.act[8];
     hd.feat = n.feat;
     hd.wb = n.wb + n.wbOff[9];
     hd.bias = n.theta + n.bOff[9];
-    hd.npix = Wp * Hp;
-    hd.H = Hp;
-    hd.W = Wp;
+    hd.Wp = Wp;
+    hd.Hp = Hp;
+    hd.w = w;
+    hd.h = h;
+    hd.passes = passes;
     hd.dst = dst;
-    hd.dstW = w;
-    hd.dstH = h;
-    hipLaunchKernelGGL(head_kernel, dim3(blocks_for((size_t)Wp * Hp)), dim3(256), 0, ctx->stream, hd);
+    hipLaunchKernelGGL(head_ensemble_kernel, dim3(blocks_for((size_t)w * h)), dim3(256), 0, ctx->stream, hd);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "net head");
     return RC_OK;
 }
@@ -1206,6 +1363,14 @@ int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t 
         out[3 * i + 1] = tmp[i].y;
         out[3 * i + 2] = tmp[i].z;
     }
+    return RC_OK;
+}
+
+int rc_set_net_flips(rc_ctx *ctx, int32_t passes) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (passes != 1 && passes != 2 && passes != 4)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "passes must be 1, 2 or 4");
+    ctx->netFlips = passes;
     return RC_OK;
 }
 

@@ -91,6 +91,11 @@ class NetDenoiser:
         self.ctx.check(self._lib.rc_net_forward(self.ctx.h, _fp(raw), n, h, w, _fp(out)))
         return out
 
+    def set_flips(self, passes: int = 1):
+        """rc_set_net_flips: network passes per denoise (1, 2 or 4), averaged over
+        the frame and its mirror images."""
+        self.ctx.check(self._lib.rc_set_net_flips(self.ctx.h, int(passes)))
+
     def enable(self, on: bool = True):
         """rc_post's denoise step runs the network (on) or a-trous (off)."""
         self.ctx.check(self._lib.rc_set_denoiser(self.ctx.h, N.RC_DENOISE_NET if on else N.RC_DENOISE_ATROUS))
