@@ -35,11 +35,15 @@
 //                       read in place — no concat or upsample buffers. Also the
 //                       input gradient: dX = W' * dZ with W' the tap-flipped,
 //                       transposed weights.
+//   conv3x3_fwd_lds<NT,MT>  the forward convolution of the three finest levels
+//                       with each block's input tile staged in LDS (same K
+//                       order and operands: bit-identical outputs).
 //   conv3x3_dw          dW[co][tap][ci] = sum_p dZ[p][co] X[p+tap][ci]: a GEMM
 //                       with K = pixels, operands transposed through LDS,
 //                       per-chunk partials reduced in a fixed order
 //                       (deterministic).
-//   head                1x1 conv + residual + expm1, loss and its gradient.
+//   head                1x1 conv + residual + expm1, loss and its gradient;
+//                       head_ensemble averages the passes of a flip ensemble.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -187,22 +191,24 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
     }
 }
 
-// The same convolution with the input staged in LDS: a block computes an 8 x 16
-// tile of output pixels of one image (wave w: rows 2w, 2w+1, 16 pixels each),
-// after loading the tile's 10 x 18 input halo, every input channel, once (zero
-// outside the image; the a source upsampled in place). The K order per output
-// (tap, then 32-channel chunk) and the operands are those of conv3x3_fwd, so
-// the outputs are bit-identical; every input pixel is read from HBM/L2 once
-// per tile instead of once per tap. LDS rows are padded by 8 channels (16 B)
+// The same convolution with the input staged in LDS: a block computes a
+// (4 MT) x 16 tile of output pixels of one image (wave w: rows MT w .. MT w +
+// MT - 1, 16 pixels each), after loading the tile's (4 MT + 2) x 18 input halo,
+// every input channel, once (zero outside the image; the a source upsampled in
+// place). The K order per output (tap, then 32-channel chunk) and the operands
+// are those of conv3x3_fwd, so the outputs are bit-identical; every input
+// pixel is read from HBM/L2 once per tile instead of once per tap, and each
+// weight fragment feeds MT MFMAs. LDS rows are padded by 8 channels (16 B)
 // against bank conflicts of the 16 pixel lanes.
-constexpr int kLdsTY = 8, kLdsTX = 16, kLdsHY = kLdsTY + 2, kLdsHX = kLdsTX + 2;
+constexpr int kLdsTX = 16, kLdsHX = kLdsTX + 2;
 
-__host__ __device__ constexpr size_t conv_lds_bytes(int cin) {
-    return (size_t)kLdsHY * kLdsHX * (size_t)(cin + 8) * 2;
+__host__ __device__ constexpr size_t conv_lds_bytes(int cin, int mt) {
+    return (size_t)(4 * mt + 2) * kLdsHX * (size_t)(cin + 8) * 2;
 }
 
-template <int NT>
+template <int NT, int MT>
 __global__ void __launch_bounds__(256) conv3x3_fwd_lds(ConvArgs A) {
+    constexpr int kLdsTY = 4 * MT, kLdsHY = kLdsTY + 2;
     extern __shared__ __align__(16) uint16_t halo[];
     const int cin = A.ca + A.cb, rs = cin + 8;
     const int tilesX = (A.W + kLdsTX - 1) / kLdsTX, tilesY = (A.H + kLdsTY - 1) / kLdsTY;
@@ -235,23 +241,25 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds(ConvArgs A) {
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kq = 8 * (lane >> 4);
-    f32x4 acc[2][NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int tap = 0; tap < 9; ++tap) {
         const int dy = tap / 3, dx = tap % 3;  // halo offsets of the tap
-        const uint16_t *r0 = halo + ((size_t)(2 * wave + dy) * kLdsHX + (lane & 15) + dx) * rs + kq;
-        const uint16_t *r1 = r0 + (size_t)kLdsHX * rs;
+        const uint16_t *r0 = halo + ((size_t)(MT * wave + dy) * kLdsHX + (lane & 15) + dx) * rs + kq;
         const uint16_t *wt = A.w + ((size_t)(lane & 15) * 9 + tap) * cin + kq;
         for (int c0 = 0; c0 < cin; c0 += 32) {
-            const bf16x8 a0 = load8(r0 + c0), a1 = load8(r1 + c0);
+            bf16x8 av[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) av[m] = load8(r0 + (size_t)m * kLdsHX * rs + c0);
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 const bf16x8 bv = load8(wt + (size_t)t * 16 * 9 * cin + c0);
-                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[0][t], 0, 0, 0);
-                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[1][t], 0, 0, 0);
+#pragma unroll
+                for (int m = 0; m < MT; ++m)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][t], 0, 0, 0);
             }
         }
     }
@@ -260,8 +268,8 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds(ConvArgs A) {
         const int co = t * 16 + (lane & 15);
         const float bb = A.bias ? A.bias[co] : 0.0f;
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int y = ty0 + 2 * wave + m;
+        for (int m = 0; m < MT; ++m) {
+            const int y = ty0 + MT * wave + m;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int x = tx0 + 4 * (lane >> 4) + r;
@@ -863,16 +871,20 @@ void launch_conv_nt(hipStream_t s, const ConvArgs &A) {
     hipLaunchKernelGGL((conv3x3_fwd<NT, 2>), dim3(grid), dim3(256), 0, s, A);
 }
 
-// conv3x3_fwd_lds for 32 or 64 output channels and at most 128 input channels
-// (LDS: 49 KB per block at 128); false when it does not apply.
+// conv3x3_fwd_lds for 32 or 64 output channels and at most 128 input channels:
+// 16 x 16 tiles (MT 4) up to 64 input channels, 8 x 16 tiles (MT 2) above
+// (LDS per block at most 49 KB); false when it does not apply.
 bool launch_conv_lds(hipStream_t s, const ConvArgs &A) {
     const int cin = A.ca + A.cb;
     if ((A.cout != 32 && A.cout != 64) || cin > 128 || (cin & 31) || (A.ca & 7) || (A.cb & 7)) return false;
-    const unsigned tiles = (unsigned)A.N * (unsigned)((A.H + kLdsTY - 1) / kLdsTY) *
+    const int mt = cin <= 64 ? 4 : 2;
+    const unsigned tiles = (unsigned)A.N * (unsigned)((A.H + 4 * mt - 1) / (4 * mt)) *
                            (unsigned)((A.W + kLdsTX - 1) / kLdsTX);
-    const size_t lds = conv_lds_bytes(cin);
-    if (A.cout == 32) hipLaunchKernelGGL(conv3x3_fwd_lds<2>, dim3(tiles), dim3(256), lds, s, A);
-    else hipLaunchKernelGGL(conv3x3_fwd_lds<4>, dim3(tiles), dim3(256), lds, s, A);
+    const size_t lds = conv_lds_bytes(cin, mt);
+    if (A.cout == 32 && mt == 4) hipLaunchKernelGGL((conv3x3_fwd_lds<2, 4>), dim3(tiles), dim3(256), lds, s, A);
+    else if (A.cout == 32) hipLaunchKernelGGL((conv3x3_fwd_lds<2, 2>), dim3(tiles), dim3(256), lds, s, A);
+    else if (mt == 4) hipLaunchKernelGGL((conv3x3_fwd_lds<4, 4>), dim3(tiles), dim3(256), lds, s, A);
+    else hipLaunchKernelGGL((conv3x3_fwd_lds<4, 2>), dim3(tiles), dim3(256), lds, s, A);
     return true;
 }
 
