@@ -458,6 +458,11 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
     }
     std::vector<float> cdf((size_t)(nLights > 0 ? nLights : 1), 1.0f);
     for (int i = 0; i < nLights; ++i) cdf[i] = lights[20 * (size_t)i + 19];
+    // a non-decreasing, NaN-free CDF (as uploadLights writes it) lets the light
+    // pick bisect instead of scanning: the first entry above u is the same
+    int cdfSorted = 1;
+    for (int i = 0; i < nLights; ++i)
+        if (cdf[i] != cdf[i] || (i > 0 && !(cdf[i - 1] <= cdf[i]))) cdfSorted = 0;
 
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
@@ -486,6 +491,7 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
     for (int k = 0; k < 3; ++k) { D.rootMin[k] = nodes[k]; D.rootMax[k] = nodes[4 + k]; }
     D.rootRef = rootRef;
     D.maxDepth = maxDepth;
+    D.cdfSorted = cdfSorted;
     D.valid = true;
     return RC_OK;
 }

@@ -73,6 +73,7 @@ struct SceneView {
     int rootRef;
     int nLights;
     float totalLightPower;
+    int cdfSorted;  // cdf is non-decreasing and NaN-free: light selection may bisect
 };
 
 // Work counters (bench instrumentation); summed per wave into global memory.

@@ -37,6 +37,7 @@ struct DeviceScene {
     int rootRef = 0;
     int maxDepth = 0;
     bool valid = false;
+    int cdfSorted = 0;  // the light CDF is non-decreasing and NaN-free
 };
 
 struct Images {
@@ -68,6 +69,7 @@ inline SceneView scene_view(const DeviceScene &D) {
     for (int k = 0; k < 3; ++k) { S.rootMin[k] = D.rootMin[k]; S.rootMax[k] = D.rootMax[k]; }
     S.rootRef = D.rootRef;
     S.nLights = D.nLights;
+    S.cdfSorted = D.cdfSorted;
     S.totalLightPower = D.totalLightPower;
     return S;
 }

@@ -257,9 +257,23 @@ __device__ __forceinline__ bool shade_vertex(const SceneView &S, int neeEnabled,
             // pdf adds thr * 0 in the reference, which leaves radiance unchanged.
             float us = rnd(seed);
             int sel = S.nLights - 1;
-            for (int i = 0; i < S.nLights; ++i) {
-                if (STATS) C.lightScans++;
-                if (us < S.cdf[i]) { sel = i; break; }
+            if (S.cdfSorted) {
+                // first i with us < cdf[i] (else the last light) by bisection: with a
+                // non-decreasing CDF the predicate is monotone, so this is the entry
+                // the reference's scan stops at; counted as that scan's length
+                int lo = 0, hi = S.nLights;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (us < S.cdf[mid]) hi = mid;
+                    else lo = mid + 1;
+                }
+                if (lo < S.nLights) sel = lo;
+                if (STATS) C.lightScans += (uint32_t)(lo < S.nLights ? lo + 1 : S.nLights);
+            } else {
+                for (int i = 0; i < S.nLights; ++i) {
+                    if (STATS) C.lightScans++;
+                    if (us < S.cdf[i]) { sel = i; break; }
+                }
             }
             if (STATS) C.lightReads++;
             const float4 *L = S.lights + 5 * (size_t)sel;

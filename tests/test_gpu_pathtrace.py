@@ -316,3 +316,25 @@ def test_invalid_inputs_are_rejected(gpu_ctx, cornell):
     buf = np.zeros(10, np.float32)
     import ctypes as C
     assert lib.rc_read(gpu_ctx.h, N.RC_BUF_ACCUM, buf.ctypes.data_as(C.c_void_p), 40) == N.RC_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("mode", [pytest.param(N.RC_PT_LANES, id="lanes"),
+                                  pytest.param(N.RC_PT_MEGAKERNEL, id="megakernel")])
+def test_unsorted_light_cdf_takes_the_scan(mode, gpu_ctx, cornell):
+    # The light pick bisects the CDF only when it is non-decreasing and NaN-free;
+    # a host that writes another CDF gets the reference's linear scan. Both the
+    # sorted (every other parity test) and the unsorted case must match the
+    # oracle, which always scans.
+    w = cornell
+    lights = w.scene.lights.copy()
+    assert lights.shape[0] >= 2
+    cdf = lights[:, 19].copy()
+    lights[:, 19] = cdf[::-1]  # decreasing: not sortable, picks change
+    sc = SceneArrays(w.scene.tris, w.scene.nodes, w.scene.mats, lights, w.scene.totalLightPower,
+                     w.scene.maxDepth, w.scene.maxLeafSize, w.scene.nLeaves)
+    cam = CameraSpec(**{**w.camera.__dict__, "width": 64, "height": 36})
+    g = gpu_render(gpu_ctx, sc, cam, w.params, 3, batch=3, mode=mode)
+    o = oracle_render(sc, cam, w.params, 3)
+    assert_parity(g, o)
+    g2 = gpu_render(gpu_ctx, w.scene, cam, w.params, 3, batch=3, mode=mode)
+    assert not np.array_equal(g["accum"], g2["accum"])  # the picks did change
