@@ -439,6 +439,10 @@ int rc_debug_pcg(rc_ctx *ctx, const int32_t *xyf, int32_t n, int32_t count, uint
  * within maxDist). n <= 2^22. */
 int rc_debug_rays(rc_ctx *ctx, const float *rays, int32_t n, int32_t *triOut, float *tOut,
                   int32_t *unoccludedOut);
+/* The triangle test's reciprocal (hardware reciprocal + one Newton step) against
+ * the IEEE division 1.0f / x for every binary32 x with 1e-8 <= |x| <= 2^126:
+ * writes the number of results that differ (0 when the fast form is exact). */
+int rc_debug_recip_check(rc_ctx *ctx, uint64_t *mismatches);
 
 #ifdef __cplusplus
 }

@@ -192,6 +192,7 @@ HIP_SYMBOLS = {
     "rc_debug_counters": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_uint64), C.c_int32]),
     "rc_debug_pcg": (C.c_int, [_VP, C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_uint32)]),
     "rc_debug_rays": (C.c_int, [_VP, _FP, C.c_int32, C.POINTER(C.c_int32), _FP, C.POINTER(C.c_int32)]),
+    "rc_debug_recip_check": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
 }
 
 HOST_SYMBOLS = {

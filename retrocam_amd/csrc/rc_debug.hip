@@ -30,7 +30,37 @@ __global__ void ray_query_kernel(SceneView S, const float4 *__restrict__ rays, i
     unocc[i] = unoccluded<false>(S, ro, rd, a.w, st, C) ? 1 : 0;
 }
 
+// Every binary32 x with 1e-8 <= |x| <= 2^126: recip_rn(x) against 1.0f / x.
+__global__ void recip_check_kernel(uint32_t base, unsigned long long *mismatches) {
+    const uint32_t i = base + (uint32_t)(blockIdx.x * 256 + threadIdx.x);
+    const float x = __uint_as_float(i);
+    if (!(fabsf(x) >= 1e-8f) || fabsf(x) > 0x1p126f) return;
+    if (__float_as_uint(recip_rn(x)) != __float_as_uint(1.0f / x)) atomicAdd(mismatches, 1ull);
+}
+
 }  // namespace rc
+
+extern "C" int rc_debug_recip_check(rc_ctx *ctx, uint64_t *mismatches) {
+    using namespace rc;
+    if (!ctx || !mismatches) return RC_ERR_INVALID_ARG;
+    if (int rc_ = capture_block(ctx, "rc_debug_recip_check")) return rc_;
+    (void)hipSetDevice(ctx->device);
+    unsigned long long *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemsetAsync(d, 0, sizeof(unsigned long long), ctx->stream);
+    const uint32_t chunk = 1u << 28;  // 16 launches of 2^28 bit patterns cover all 2^32
+    for (uint32_t k = 0; k < 16 && e == hipSuccess; ++k) {
+        hipLaunchKernelGGL(recip_check_kernel, dim3(chunk / 256), dim3(256), 0, ctx->stream, k * chunk, d);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (d) (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rc_debug_recip_check");
+    *mismatches = h;
+    return RC_OK;
+}
 
 extern "C" int rc_debug_rays(rc_ctx *ctx, const float *rays, int32_t n, int32_t *triOut, float *tOut,
                              int32_t *unoccludedOut) {

@@ -101,6 +101,17 @@ __device__ __forceinline__ bool aabb(v3 ro, v3 inv, float mnx, float mny, float 
     return tx >= fmaxf(tn, 0.0f);
 }
 
+// 1/x rounded to nearest, as the IEEE division computes it, for the x the
+// triangle test divides by (|x| >= 1e-8): v_rcp_f32 and one FMA Newton step
+// give the correctly rounded reciprocal for every finite |x| <= 2^126 (checked
+// on the hardware for all such binary32 values, rc_debug_recip_check); above
+// that the result is subnormal and the IEEE division runs.
+__device__ __forceinline__ float recip_rn(float x) {
+    if (__builtin_expect(fabsf(x) > 0x1p126f, 0)) return 1.0f / x;
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+}
+
 // intersectTriangle (pathtrace.comp:112-130) on pre-subtracted edges.
 __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int idx,
                                          float &t, float &bu, float &bv) {
@@ -110,7 +121,7 @@ __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int i
     v3 h = cross(rd, e2);
     float det = dot(e1, h);
     if (fabsf(det) < 1e-8f) return false;
-    float f = 1.0f / det;
+    float f = recip_rn(det);
     v3 s = ro - v0;
     float u = f * dot(s, h);
     if (u < 0.0f || u > 1.0f) return false;

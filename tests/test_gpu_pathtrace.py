@@ -338,3 +338,13 @@ def test_unsorted_light_cdf_takes_the_scan(mode, gpu_ctx, cornell):
     assert_parity(g, o)
     g2 = gpu_render(gpu_ctx, w.scene, cam, w.params, 3, batch=3, mode=mode)
     assert not np.array_equal(g["accum"], g2["accum"])  # the picks did change
+
+
+def test_triangle_reciprocal_is_exact(gpu_ctx):
+    # The triangle test divides by det with the hardware reciprocal and one FMA
+    # Newton step below |det| = 2^126 (IEEE division above): every binary32 det
+    # it can see must give the correctly rounded 1/det, as the oracle's division.
+    import ctypes as C
+    m = C.c_uint64(123)
+    gpu_ctx.check(gpu_ctx._lib.rc_debug_recip_check(gpu_ctx.h, C.byref(m)))
+    assert m.value == 0
