@@ -18,10 +18,6 @@ F = np.float32
 _MAXF = Fraction(2) ** 128 - Fraction(2) ** 103  # halfway between FLT_MAX and 2^128 (ties go up: odd)
 
 
-def _f32_bits(x):
-    return int(np.array(x, F).view(np.uint32))
-
-
 def round_f32(q: Fraction):
     """The binary32 nearest to q, ties to even (overflow to infinity)."""
     if q == 0:
