@@ -50,7 +50,10 @@ int32_t rcs_rotate_since(rcs_scene *s, int32_t first, float px, float py, float 
 int32_t rcs_add_triangles(rcs_scene *s, const float *tris32, int32_t n);
 int32_t rcs_triangle_count(const rcs_scene *s);
 
-/* Builds the BVH (reordering triangles) and the light list. Returns 0 or -1. */
+/* Builds the BVH (reordering triangles) and the light list. Returns 0 or -1;
+ * -1 also where BVHBuilder.java:117-119 throws (a bucket index below 0, from a
+ * centroid that rounds under a thin node's minimum), with the triangles left in
+ * their original order. */
 int32_t rcs_build(rcs_scene *s);
 
 typedef struct rcs_info {

@@ -105,9 +105,22 @@ constexpr int kLeafMax = 4;
 constexpr float kCTrav = 1.0f;
 constexpr float kCIsect = 1.5f;
 
+// Java's (int) cast of a float: NaN -> 0, saturating at the int range, else
+// truncation toward zero.
+inline int64_t java_int(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return -2147483647 - 1;
+    return (int64_t)f;
+}
+
 struct Builder {  // BVHBuilder (BVHBuilder.java:18-164)
     std::vector<Tri> &tris;
     std::vector<Node> nodes;
+    // A centroid that rounds below its node's minimum by at least an eighth of
+    // a thin (<= 8 ulp) extent gives a bucket index <= -1: BVHBuilder.java:117-119
+    // indexes buckets[-1] and throws, so the build fails here as well.
+    bool failed = false;
     explicit Builder(std::vector<Tri> &t) : tris(t) {}
 
     Node from_tris(int start, int count) const {
@@ -121,7 +134,7 @@ struct Builder {  // BVHBuilder (BVHBuilder.java:18-164)
         return n;
     }
 
-    bool best_split(const Node &p, int start, int count, int &bAxis, float &bPos, float &bCost) const {
+    bool best_split(const Node &p, int start, int count, int &bAxis, float &bPos, float &bCost) {
         float parentSA = p.sa();
         if (parentSA < 1e-8f) return false;
         bool found = false;
@@ -132,8 +145,9 @@ struct Builder {  // BVHBuilder (BVHBuilder.java:18-164)
             for (int i = start; i < start + count; ++i) {
                 float c = tris[i].centroid(axis);
                 float f = ((float)kBuckets * (c - lo)) / (hi - lo);
-                int k = (int)f;  // truncation toward zero, as Java's (int)
+                int64_t k = java_int(f);
                 if (k > kBuckets - 1) k = kBuckets - 1;
+                if (k < 0) { failed = true; return false; }
                 b[k].count++;
                 b[k].expand(tris[i]);
             }
@@ -171,11 +185,13 @@ struct Builder {  // BVHBuilder (BVHBuilder.java:18-164)
         if (count <= kLeafMax) return make_leaf();
         int axis = 0; float pos = 0.0f, cost = 0.0f;
         bool ok = best_split(nodes[idx], start, count, axis, pos, cost);
+        if (failed) return idx;
         float leafCost = kCIsect * (float)count;
         if (!ok || cost >= leafCost) return make_leaf();
         int mid = partition(start, count, axis, pos);
         if (mid == start || mid == start + count) return make_leaf();
         int l = subdivide(start, mid - start);
+        if (failed) return idx;
         int r = subdivide(mid, start + count - mid);
         nodes[idx].left = l;
         nodes[idx].right = r;
@@ -368,8 +384,13 @@ int32_t rcs_build(rcs_scene *s) {
     int nMats = (int)(s->mats.size() / 16);
     for (const Tri &t : s->tris)
         if (t.mat < 0 || t.mat >= nMats) return -1;
-    Builder b(s->tris);
-    b.subdivide(0, (int)s->tris.size());
+    // BVHBuilder.build reorders a copy of the list (BVHBuilder.java:35-37): a
+    // failed build leaves the scene's triangles as they were.
+    std::vector<Tri> order = s->tris;
+    Builder b(order);
+    b.subdivide(0, (int)order.size());
+    if (b.failed) return -1;
+    s->tris.swap(order);
     s->nodes.swap(b.nodes);
     depth_walk(s->nodes, s);
 
