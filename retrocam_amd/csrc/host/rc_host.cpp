@@ -178,24 +178,40 @@ struct Builder {  // BVHBuilder (BVHBuilder.java:18-164)
         return lo;
     }
 
-    int subdivide(int start, int count) {
-        int idx = (int)nodes.size();
-        nodes.push_back(from_tris(start, count));
-        auto make_leaf = [&]() { nodes[idx].primStart = start; nodes[idx].primCount = count; return idx; };
-        if (count <= kLeafMax) return make_leaf();
+    // BVHBuilder.subdivide (BVHBuilder.java:61-95) as a depth-first walk over an
+    // explicit stack: nodes are numbered in the same pre-order (a node, its left
+    // subtree, its right subtree) and ranges are partitioned in the same order.
+    void build_all(int count) {
+        struct Task { int start, count, parent, right; };
+        std::vector<Task> todo;
+        todo.push_back({0, count, -1, 0});
+        while (!todo.empty() && !failed) {
+            Task t = todo.back();
+            todo.pop_back();
+            int idx = (int)nodes.size();
+            nodes.push_back(from_tris(t.start, t.count));
+            if (t.parent >= 0) (t.right ? nodes[t.parent].right : nodes[t.parent].left) = idx;
+            int mid = split(idx, t.start, t.count);
+            if (mid < 0) {
+                nodes[idx].primStart = t.start;
+                nodes[idx].primCount = t.count;
+                continue;
+            }
+            todo.push_back({mid, t.start + t.count - mid, idx, 1});
+            todo.push_back({t.start, mid - t.start, idx, 0});
+        }
+    }
+
+    // The partition point of node idx over [start, start+count), or -1 for a leaf.
+    int split(int idx, int start, int count) {
+        if (count <= kLeafMax) return -1;
         int axis = 0; float pos = 0.0f, cost = 0.0f;
         bool ok = best_split(nodes[idx], start, count, axis, pos, cost);
-        if (failed) return idx;
         float leafCost = kCIsect * (float)count;
-        if (!ok || cost >= leafCost) return make_leaf();
+        if (failed || !ok || cost >= leafCost) return -1;
         int mid = partition(start, count, axis, pos);
-        if (mid == start || mid == start + count) return make_leaf();
-        int l = subdivide(start, mid - start);
-        if (failed) return idx;
-        int r = subdivide(mid, start + count - mid);
-        nodes[idx].left = l;
-        nodes[idx].right = r;
-        return idx;
+        if (mid == start || mid == start + count) return -1;
+        return mid;
     }
 };
 
@@ -388,7 +404,7 @@ int32_t rcs_build(rcs_scene *s) {
     // failed build leaves the scene's triangles as they were.
     std::vector<Tri> order = s->tris;
     Builder b(order);
-    b.subdivide(0, (int)order.size());
+    b.build_all((int)order.size());
     if (b.failed) return -1;
     s->tris.swap(order);
     s->nodes.swap(b.nodes);
