@@ -1,17 +1,20 @@
-"""CPU oracle of the scene's BVH: the binned-SAH builder of BVHBuilder
-(src/main/java/com/retrocam/scene/BVHBuilder.java:35-202), the node bounds of
-BVHNode (src/main/java/com/retrocam/scene/BVHNode.java:44-61) and the triangle
-bounds and centroids of GpuTriangle
-(src/main/java/com/retrocam/scene/GpuTriangle.java:53-65).
+"""CPU oracle of the scene's BVH and light list: the binned-SAH builder of
+BVHBuilder (src/main/java/com/retrocam/scene/BVHBuilder.java:35-202), the node
+bounds of BVHNode (src/main/java/com/retrocam/scene/BVHNode.java:44-61), the
+triangle bounds and centroids of GpuTriangle
+(src/main/java/com/retrocam/scene/GpuTriangle.java:53-65), and the light records
+and CDF of SceneUploader.uploadLights
+(src/main/java/com/retrocam/scene/SceneUploader.java:144-216).
 
 TEST INFRASTRUCTURE ONLY: used by tests/ to check librc_host.so's rcs_build
-node for node. Nothing in the product imports it.
+record for record. Nothing in the product imports it.
 
 Java semantics, one operation at a time: float arithmetic is binary32 (every
 intermediate is a numpy.float32), an int operand of a float product is
 converted first, (int) of a float is NaN -> 0, saturating, else truncation
 toward zero, Math.min / Math.max return NaN when an operand is NaN and order
--0.0 below +0.0, and an array index below 0 throws (here BucketIndexError).
+-0.0 below +0.0, (float) Math.sqrt of a float is the correctly rounded binary32
+square root, and an array index below 0 throws (here BucketIndexError).
 """
 import numpy as np
 
@@ -195,3 +198,44 @@ def nodes_to_records(nodes):
         ints[i, 10] = n["primStart"]
         ints[i, 11] = n["primCount"]
     return out
+
+
+def lights(tris32, mats16):
+    """uploadLights over 32-float triangle records in BVH order and 16-float
+    material records. Returns ((n, 20) float32 light records, total power).
+    Overflow to inf and inf / inf = NaN follow IEEE arithmetic, as in Java."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        return _lights(tris32, mats16)
+
+
+def _lights(tris32, mats16):
+    pi_f = F(np.pi)
+    out, total = [], F(0.0)
+    for t in np.asarray(tris32, np.float32):
+        m = mats16[int(t[30:31].view(np.int32)[0])]
+        strength = F(m[7])
+        if strength <= F(0.0):
+            continue
+        v0, v1, v2 = t[0:3], t[4:7], t[8:11]
+        e1 = [F(v1[a]) - F(v0[a]) for a in range(3)]
+        e2 = [F(v2[a]) - F(v0[a]) for a in range(3)]
+        cx = e1[1] * e2[2] - e1[2] * e2[1]
+        cy = e1[2] * e2[0] - e1[0] * e2[2]
+        cz = e1[0] * e2[1] - e1[1] * e2[0]
+        area = F(0.5) * np.sqrt(cx * cx + cy * cy + cz * cz, dtype=np.float32)
+        if area < F(1e-8):
+            continue
+        er, eg, eb = F(m[4]) * strength, F(m[5]) * strength, F(m[6]) * strength
+        lum = F(0.299) * er + F(0.587) * eg + F(0.114) * eb
+        power = lum * area * pi_f
+        total = total + power
+        out.append([v0[0], v0[1], v0[2], area, v1[0], v1[1], v1[2], 0, v2[0], v2[1], v2[2], 0,
+                    er, eg, eb, power, t[12], t[13], t[14], 0])
+    rec = np.array(out, np.float32).reshape(-1, 20)
+    cum = F(0.0)
+    for e in rec:
+        cum = cum + e[15]
+        e[19] = cum / total if total > F(0.0) else F(1.0)
+    if len(rec):
+        rec[-1, 19] = F(1.0)
+    return rec, total

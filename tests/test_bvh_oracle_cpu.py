@@ -1,10 +1,13 @@
-"""The binned-SAH BVH of librc_host.so (rcs_build) against the oracle's
-restatement of BVHBuilder (src/main/java/com/retrocam/scene/BVHBuilder.java:35-202),
-node for node and in triangle order, bit for bit.
+"""The binned-SAH BVH and the light list of librc_host.so (rcs_build) against
+the oracle's restatements of BVHBuilder
+(src/main/java/com/retrocam/scene/BVHBuilder.java:35-202) and
+SceneUploader.uploadLights (src/main/java/com/retrocam/scene/SceneUploader.java:144-216),
+node for node, light for light and in triangle order, bit for bit.
 
-The oracle is pinned by hand-worked builds: bucket assignment, split costs,
-the partition's swap order, the axis tie rule, the leaf rules and the
-exception the Java builder raises on a thin node."""
+The oracle is pinned by hand-worked builds (bucket assignment, split costs, the
+partition's swap order, the axis tie rule, the leaf rules, the exception the
+Java builder raises on a thin node) and by closed-form light records (areas,
+powers and CDFs of equal lights, the skip rules)."""
 import os
 import sys
 
@@ -19,23 +22,31 @@ import rc_oracle_bvh as ob  # noqa: E402
 f32 = np.float32
 
 
-def _records(verts):
-    """32-float triangle records; uv0.x carries the input index."""
+def _records(verts, mat=None):
+    """32-float triangle records; uv0.x carries the input index, normals +z."""
     n = len(verts)
     rec = np.zeros((n, 32), f32)
     v = np.asarray(verts, f32)
     rec[:, 0:3], rec[:, 4:7], rec[:, 8:11] = v[:, 0], v[:, 1], v[:, 2]
     rec[:, 14] = rec[:, 18] = rec[:, 22] = 1.0
     rec[:, 24] = np.arange(n, dtype=f32)
-    rec.view(np.int32)[:, 30] = 0
+    rec.view(np.int32)[:, 30] = 0 if mat is None else np.asarray(mat, np.int32)
     return rec
 
 
-def _host_build(verts):
+def _host_scene(verts, mats=(Material.diffuse(0.5, 0.5, 0.5),), mat=None, normals=None):
     b = SceneBuilder()
-    b.add_material(Material.diffuse(0.5, 0.5, 0.5))
-    b.add_triangles(_records(verts))
-    sc = b.build()
+    for m in mats:
+        b.add_material(m)
+    rec = _records(verts, mat)
+    if normals is not None:  # (n0, n1, n2) for every triangle
+        rec[:, 12:15], rec[:, 16:19], rec[:, 20:23] = normals
+    b.add_triangles(rec)
+    return b.build()
+
+
+def _host_build(verts):
+    sc = _host_scene(verts)
     return sc.nodes, sc.tris[:, 24].astype(np.int64).tolist()
 
 
@@ -247,3 +258,90 @@ def test_cornell_scene_matches_oracle():
     from retrocam_amd import workloads
     t = workloads.cornell_small().scene.tris
     _check_same(np.stack([t[:, 0:3], t[:, 4:7], t[:, 8:11]], 1))
+
+
+def _check_lights(sc):
+    """The host's light records and total power equal the oracle's, bit for bit."""
+    rec, total = ob.lights(sc.tris, sc.mats)
+    assert sc.nLights == len(rec)
+    if len(rec):
+        assert np.array_equal(sc.lights[:len(rec)].view(np.uint32), rec.view(np.uint32))
+    assert f32(sc.totalLightPower).view(np.uint32) == total.view(np.uint32)
+    return rec, total
+
+
+def _right(x0, z=0.0):
+    """Right triangle with legs 2 and 2 in the plane z: area exactly 2."""
+    return [(x0, 0, z), (x0 + 2, 0, z), (x0, 2, z)]
+
+
+def test_light_record_closed_form():
+    # Area 0.5 * sqrt(4^2) = 2; emission (1, 0, 0) x strength 1: lum = 0.299f,
+    # power = (0.299f * 2) * (float) pi with one rounding (the doubling is
+    # exact). One light: CDF 1, total = its power; the normal is n0's.
+    sc = _host_scene([_right(0)], mats=[Material(emission=(1.0, 0.0, 0.0), emissionStrength=1.0)],
+                     normals=((0, 0, 1), (1, 0, 0), (0, 1, 0)))
+    rec, total = _check_lights(sc)
+    assert rec.shape == (1, 20) and rec[0, 3] == 2.0
+    assert rec[0, 15] == f32(f32(0.598) * f32(np.pi)) and total == rec[0, 15] and rec[0, 19] == 1.0
+    assert list(rec[0, 16:19]) == [0, 0, 1] and list(rec[0, 12:15]) == [1, 0, 0]
+
+
+def test_equal_lights_cdf():
+    # Two equal lights: cumulative / sum = p / 2p = 0.5 exactly (2p is exact),
+    # then 1. Four: 0.25 and 0.5 exactly, 3p / 4p within an ulp of 0.75 (3p
+    # rounds), and the last entry forced to 1.
+    em = Material(emission=(1.0, 1.0, 1.0), emissionStrength=4.0)
+    rec, total = _check_lights(_host_scene([_right(0), _right(10)], mats=[em]))
+    assert list(rec[:, 19]) == [0.5, 1.0] and total == rec[0, 15] * 2
+    rec, total = _check_lights(_host_scene([_right(10 * k) for k in range(4)], mats=[em]))
+    assert np.all(rec[:, 15] == rec[0, 15])
+    assert list(rec[[0, 1, 3], 19]) == [0.25, 0.5, 1.0] and abs(rec[2, 19] - 0.75) <= 2.0 ** -24
+
+
+def test_overflowing_power_cdf():
+    # A strength of 3e38 overflows the power to +inf: the total is inf, the first
+    # entry's CDF p / inf = 0 and the last one's inf / inf = NaN, replaced by 1.
+    em = Material(emission=(1.0, 1.0, 1.0), emissionStrength=1.0)
+    hot = Material(emission=(1.0, 1.0, 1.0), emissionStrength=3e38)
+    sc = _host_scene([_right(0), _right(10)], mats=[em, hot], mat=[0, 1])
+    rec, total = _check_lights(sc)
+    assert np.isinf(total) and sorted(rec[:, 19].tolist()) == [0.0, 1.0]
+
+
+def test_light_skip_rules():
+    em = Material(emission=(1.0, 1.0, 1.0), emissionStrength=1.0)
+    off = Material(emission=(1.0, 1.0, 1.0), emissionStrength=0.0)
+    neg = Material(emission=(1.0, 1.0, 1.0), emissionStrength=-1.0)
+    black = Material(emission=(0.0, 0.0, 0.0), emissionStrength=1.0)
+    # strength <= 0 -> not a light; a collinear (zero-area) emitter -> skipped
+    needle = [(0, 0, 0), (1, 1, 1), (2, 2, 2)]
+    sc = _host_scene([_right(0), _right(5), needle, _right(9)], mats=[em, off, neg], mat=[1, 2, 0, 0])
+    rec, _ = _check_lights(sc)
+    assert len(rec) == 1 and rec[0, 0] == 9.0
+    # zero luminance but positive strength: a light of power 0; all powers 0 -> CDF 1
+    rec, total = _check_lights(_host_scene([_right(0), _right(5)], mats=[black]))
+    assert len(rec) == 2 and total == 0 and list(rec[:, 19]) == [1.0, 1.0]
+    # no emitter at all
+    sc = _host_scene([_right(0)], mats=[off])
+    assert sc.nLights == 0 and sc.totalLightPower == 0
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_random_emitters_match_oracle(seed):
+    # Lights are listed in BVH triangle order with their CDF.
+    rng = np.random.default_rng(seed)
+    n = 120
+    verts = _random_scene(rng, n, "uniform")
+    mats = [Material.diffuse(0.5, 0.5, 0.5)] + [
+        Material(emission=tuple(rng.uniform(0, 2, 3)), emissionStrength=float(rng.uniform(0.1, 20)))
+        for _ in range(5)]
+    sc = _host_scene(verts, mats=mats, mat=rng.integers(0, 6, n))
+    rec, _ = _check_lights(sc)
+    assert len(rec) > 10
+
+
+def test_workload_lights_match_oracle():
+    from retrocam_amd import workloads
+    for w in (workloads.cornell_small(), workloads.sppm_caustics()):
+        _check_lights(w.scene)
