@@ -2,7 +2,7 @@
 against the CPU oracle on the same scenes, seeds and sample counts.
 
 Bars: PCG streams, primary-hit and G-buffer triangle IDs and the binary16
-G-buffers bit-exact; per-pixel radiance relMSE <= 1e-3 (BASELINE acceptance),
+G-buffers bit-exact; per-pixel radiance relMSE <= 1e-3 (the path's stated tolerance),
 and in practice bit-identical (both sides run the same IEEE binary32 operation
 sequence), which is asserted on >= 99.9% of pixels."""
 import json

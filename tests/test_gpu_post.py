@@ -1,5 +1,5 @@
 """GPU parity of the post chain (p00, a-trous, p01..p16, display) through the C
-ABI against the CPU oracle: outputs within 1e-5 absolute (BASELINE acceptance
+ABI against the CPU oracle: outputs within 1e-5 absolute (the stated tolerance
 for a-trous and VHS), RGBA8 display values equal up to one code on >= 99.9%."""
 import numpy as np
 import pytest
