@@ -141,7 +141,8 @@ class SceneBuilder:
 
     def build(self) -> SceneArrays:
         if self._lib.rcs_build(self._s) != 0:
-            raise RuntimeError("rcs_build failed (empty scene or bad material index)")
+            raise RuntimeError("rcs_build failed: empty scene, a bad material index, or a node only a few "
+                               "ulps thick whose bucket index falls below 0 (where BVHBuilder throws)")
         info = N.RcsInfo()
         self._lib.rcs_get_info(self._s, C.byref(info))
         tris = np.zeros((info.nTris, 32), dtype=f32)
