@@ -30,7 +30,7 @@
 // operands), the head and the optimizer (Adam on FP32 master weights) are FP32.
 //
 // Kernels:
-//   conv3x3_fwd<NT,MT>  out = act(W * [a | b] + bias); the input is the channel
+//   conv3x3_fwd<NT,MT,G> out = act(W * [a | b] + bias); the input is the channel
 //                       concatenation of a (optionally nearest-upsampled) and b,
 //                       read in place — no concat or upsample buffers. Also the
 //                       input gradient: dX = W' * dZ with W' the tap-flipped,
@@ -99,7 +99,10 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u3
 // ---------------------------------------------------------------------------
 // 3x3 convolution, implicit GEMM: M = output pixels, N = output channels,
 // K = 9 taps x input channels (multiple of 32). A wave computes 16*MT pixels x
-// 16*NT channels. MFMA 16x16x32 operand layout (lane l): A row l%16, k 8*(l/16)+j;
+// 16*NT channels; the 4 waves of a block cover 4/G pixel tiles x G channel
+// groups (cout = 16 NT G), so a small level can run G times as many waves.
+// Each output's K order (tap, then 32-channel chunk) does not depend on NT, MT
+// or G. MFMA 16x16x32 operand layout (lane l): A row l%16, k 8*(l/16)+j;
 // B column l%16, k 8*(l/16)+j; D column l%16, rows 4*(l/16)+r.
 struct ConvArgs {
     const uint16_t *a;  // [pixels of a][ca]   (a at half resolution when aUp)
@@ -113,11 +116,13 @@ struct ConvArgs {
     float *outF;        // f32 output
 };
 
-template <int NT, int MT>
+template <int NT, int MT, int G>
 __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
+    static_assert(G == 1 || G == 2 || G == 4, "channel groups per block");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int npix = A.N * A.H * A.W;
-    const int base = (blockIdx.x * 4 + wave) * 16 * MT;
+    const int base = (blockIdx.x * (4 / G) + wave / G) * 16 * MT;
+    const int co0 = (wave % G) * 16 * NT;  // this wave's first output channel
     if (base >= npix) return;
     const int cin = A.ca + A.cb;
     const int kq = 8 * (lane >> 4);  // this lane's k offset inside a 32-wide chunk
@@ -155,7 +160,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
             srcA[m] = A.a + pa * A.ca + kq;
             srcB[m] = A.b ? A.b + (((size_t)pn[m] * A.H + ys) * A.W + xs) * A.cb + kq : nullptr;
         }
-        const uint16_t *wt = A.w + ((size_t)(lane & 15) * 9 + tap) * cin + kq;
+        const uint16_t *wt = A.w + ((size_t)(co0 + (lane & 15)) * 9 + tap) * cin + kq;
         for (int c0 = 0; c0 < cin; c0 += 32) {
             bf16x8 av[MT];
 #pragma unroll
@@ -174,7 +179,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + (lane & 15);
+        const int co = co0 + t * 16 + (lane & 15);
         const float bb = A.bias ? A.bias[co] : 0.0f;
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
@@ -864,12 +869,17 @@ hipError_t ensure_ws(rc_ctx *ctx, Net &n, int N, int H, int W, bool train) {
 
 unsigned blocks_for(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
-template <int NT>
+template <int NT, int G>
 void launch_conv_nt(hipStream_t s, const ConvArgs &A) {
     const size_t npix = (size_t)A.N * A.H * A.W;
-    const unsigned grid = (unsigned)((npix + 127) / 128);  // 4 waves x 32 pixels
-    hipLaunchKernelGGL((conv3x3_fwd<NT, 2>), dim3(grid), dim3(256), 0, s, A);
+    const size_t perBlock = (size_t)32 * (4 / G);  // 4/G pixel tiles of 32
+    const unsigned grid = (unsigned)((npix + perBlock - 1) / perBlock);
+    hipLaunchKernelGGL((conv3x3_fwd<NT, 2, G>), dim3(grid), dim3(256), 0, s, A);
 }
+
+// Below this many 32-pixel tiles (4 waves per SIMD on 256 CUs) a layer splits
+// its output channels over the waves of a block.
+constexpr size_t kFewTiles = 4096;
 
 // conv3x3_fwd_lds for 32 or 64 output channels and at most 128 input channels:
 // 16 x 16 tiles (MT 4) up to 64 input channels, 8 x 16 tiles (MT 2) above
@@ -889,12 +899,14 @@ bool launch_conv_lds(hipStream_t s, const ConvArgs &A) {
 }
 
 hipError_t launch_conv(hipStream_t s, const ConvArgs &A) {
+    if (A.cout % 16) return hipErrorInvalidValue;
+    const bool few = ((size_t)A.N * A.H * A.W + 31) / 32 < kFewTiles;
     switch (A.cout / 16) {
-        case 2: launch_conv_nt<2>(s, A); break;
-        case 4: launch_conv_nt<4>(s, A); break;
-        case 6: launch_conv_nt<6>(s, A); break;
-        case 8: launch_conv_nt<8>(s, A); break;
-        case 14: launch_conv_nt<14>(s, A); break;
+        case 2: few ? launch_conv_nt<1, 2>(s, A) : launch_conv_nt<2, 1>(s, A); break;
+        case 4: few ? launch_conv_nt<1, 4>(s, A) : launch_conv_nt<4, 1>(s, A); break;
+        case 6: few ? launch_conv_nt<3, 2>(s, A) : launch_conv_nt<6, 1>(s, A); break;
+        case 8: few ? launch_conv_nt<2, 4>(s, A) : launch_conv_nt<8, 1>(s, A); break;
+        case 14: few ? launch_conv_nt<7, 2>(s, A) : launch_conv_nt<14, 1>(s, A); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
