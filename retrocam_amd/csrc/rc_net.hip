@@ -403,20 +403,42 @@ __global__ void reduce_dw(const float *part, int chunks, int cout, int cinPad, i
 }
 
 // Bias gradient: per-chunk column sums of dZ, then a fixed-order reduction.
-__global__ void colsum_part(const uint16_t *dz, int npix, int c, int pixPerChunk, float *part) {
-    const int ch = threadIdx.x;
-    if (ch >= c) return;
+// Bias gradient, stage 1: column sums of a bf16 [npix][c] tensor over chunks
+// of pixPerChunk pixels (c <= 256) into part[chunk][c]. Thread t of the block
+// takes channel t % c of pixel phase t / c (256 / c phases, so a wave reads
+// whole pixel rows), sums its phase's pixels in order, and the phases are
+// added in order through LDS.
+__global__ void __launch_bounds__(256) colsum_part(const uint16_t *dz, int npix, int c, int pixPerChunk,
+                                                   float *part) {
+    __shared__ float red[256];
+    const int t = threadIdx.x, phases = 256 / c;
+    const int ch = t % c, j = t / c;
     const int p0 = blockIdx.x * pixPerChunk, p1 = min(npix, p0 + pixPerChunk);
     float s = 0.0f;
-    for (int p = p0; p < p1; ++p) s += bf2f(dz[(size_t)p * c + ch]);
-    part[(size_t)blockIdx.x * c + ch] = s;
+    if (j < phases)
+        for (int p = p0 + j; p < p1; p += phases) s += bf2f(dz[(size_t)p * c + ch]);
+    red[t] = s;
+    __syncthreads();
+    if (t < c) {
+        float r = red[t];
+        for (int k = 1; k < phases; ++k) r += red[k * c + t];
+        part[(size_t)blockIdx.x * c + t] = r;
+    }
 }
-__global__ void reduce_cols(const float *part, int chunks, int c, float *g) {
-    const int ch = threadIdx.x;
-    if (ch >= c) return;
+// Stage 2: one block per channel; thread t sums chunks t, t + 256, ... in
+// order, then a fixed LDS tree adds the 256 partial sums.
+__global__ void __launch_bounds__(256) reduce_cols(const float *part, int chunks, int c, float *g) {
+    __shared__ float red[256];
+    const int t = threadIdx.x, ch = blockIdx.x;
     float s = 0.0f;
-    for (int k = 0; k < chunks; ++k) s += part[(size_t)k * c + ch];
-    g[ch] = s;
+    for (int k = t; k < chunks; k += 256) s += part[(size_t)k * c + ch];
+    red[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) g[ch] = red[0];
 }
 
 // 2x2 mean pooling of a bf16 NHWC tensor (H, W of the input, even).
@@ -974,8 +996,10 @@ hipError_t weight_grad(rc_ctx *ctx, Net &n, int L, int N, int H, int W) {
     hipLaunchKernelGGL(reduce_dw, dim3(blocks_for(cnt)), dim3(256), 0, s, n.part, (int)chunks, sp.cout, sp.cinPad,
                        sp.cinReal, n.grad + n.pOff[L]);
     const unsigned cchunks = blocks_for(np, 256);
-    hipLaunchKernelGGL(colsum_part, dim3(cchunks), dim3(128), 0, s, n.dz[L], (int)np, sp.cout, 256, n.part);
-    hipLaunchKernelGGL(reduce_cols, dim3(1), dim3(128), 0, s, n.part, (int)cchunks, sp.cout, n.grad + n.bOff[L]);
+    if (sp.cout > 256) return hipErrorInvalidValue;  // colsum_part holds a channel per thread
+    hipLaunchKernelGGL(colsum_part, dim3(cchunks), dim3(256), 0, s, n.dz[L], (int)np, sp.cout, 256, n.part);
+    hipLaunchKernelGGL(reduce_cols, dim3(sp.cout), dim3(256), 0, s, n.part, (int)cchunks, sp.cout,
+                       n.grad + n.bOff[L]);
     return hipGetLastError();
 }
 

@@ -64,6 +64,10 @@ def test_gradient_matches_oracle(gpu_ctx):
         cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
         rel = float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30))
         assert cos >= 0.999 and rel <= 3e-2, (name, cos, rel)
+        # the bias block on its own (column sums of the layer's output gradient)
+        ab, rb = a[-bs[0]:], b[-bs[0]:]
+        rel_b = float(np.linalg.norm(ab - rb) / (np.linalg.norm(rb) + 1e-30))
+        assert rel_b <= 3e-2, (name, "bias", rel_b)
 
 
 def test_training_reduces_loss(gpu_ctx):
