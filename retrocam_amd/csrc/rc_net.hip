@@ -291,9 +291,10 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds(ConvArgs A) {
 
 // ---------------------------------------------------------------------------
 // Weight gradient of a 3x3 convolution. Block (chunk, tap): pixels
-// [chunk*pixPerChunk, +pixPerChunk) in steps of 32 (the MFMA K); dZ and the
-// tap-shifted input are staged in LDS transposed ([channel][pixel]) so that
-// each lane's 8 consecutive K values are one 16-byte LDS read.
+// [chunk*pixPerChunk, +pixPerChunk) staged kDwStep at a time and fed to the
+// MFMAs as 32-pixel K chunks in pixel order; dZ and the tap-shifted input are
+// staged in LDS transposed ([channel][pixel]) so that each lane's 8
+// consecutive K values are one 16-byte LDS read.
 struct DwArgs {
     const uint16_t *dz;  // [npix][cout]
     const uint16_t *a;
@@ -304,7 +305,8 @@ struct DwArgs {
     int pixPerChunk;
     float *part;         // [chunks][cout][9][cin]
 };
-constexpr int kDwPad = 40;     // LDS row length in bf16 (32 + 8: 80-byte rows, 16-byte aligned)
+constexpr int kDwStep = 64;    // pixels staged per LDS round: two 32-pixel MFMA K chunks
+constexpr int kDwPad = kDwStep + 8;  // LDS row length in bf16 (144-byte rows, 16-byte aligned)
 constexpr int kDwMaxTiles = 16;
 
 __global__ void __launch_bounds__(256) conv3x3_dw(DwArgs A) {
@@ -323,10 +325,10 @@ __global__ void __launch_bounds__(256) conv3x3_dw(DwArgs A) {
     f32x4 acc[kDwMaxTiles];
 #pragma unroll
     for (int i = 0; i < kDwMaxTiles; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int ps = p0; ps < p1; ps += 32) {
-        // stage dZ[ps..ps+31][cout] -> dzs[co][px]
+    for (int ps = p0; ps < p1; ps += kDwStep) {
+        // stage dZ[ps..ps+kDwStep-1][cout] -> dzs[co][px]
         const int gz = A.cout / 8;
-        for (int i = threadIdx.x; i < 32 * gz; i += 256) {
+        for (int i = threadIdx.x; i < kDwStep * gz; i += 256) {
             int pxl = i / gz, cg = i % gz;
             int p = ps + pxl;
             u32x4 v = u32x4{0u, 0u, 0u, 0u};
@@ -337,7 +339,7 @@ __global__ void __launch_bounds__(256) conv3x3_dw(DwArgs A) {
         }
         // stage X[p + tap][cin] -> xs[ci][px]
         const int gx = cin / 8;
-        for (int i = threadIdx.x; i < 32 * gx; i += 256) {
+        for (int i = threadIdx.x; i < kDwStep * gx; i += 256) {
             int pxl = i / gx, cg = i % gx;
             int p = ps + pxl;
             u32x4 v = u32x4{0u, 0u, 0u, 0u};
@@ -360,14 +362,21 @@ __global__ void __launch_bounds__(256) conv3x3_dw(DwArgs A) {
             for (int j = 0; j < 8; ++j) xs[(cg * 8 + j) * kDwPad + pxl] = h[j];
         }
         __syncthreads();
+        // per tile, the 32-pixel K chunks in pixel order (a chunk wholly past p1 is skipped)
 #pragma unroll
         for (int i = 0; i < kDwMaxTiles; ++i) {
             const int tile = wave + 4 * i;
             if (tile < ntiles) {
                 const int mt = tile / ntc, nt = tile % ntc;
-                bf16x8 av = __builtin_bit_cast(bf16x8, *(const u32x4 *)&dzs[(mt * 16 + (lane & 15)) * kDwPad + kq]);
-                bf16x8 bv = __builtin_bit_cast(bf16x8, *(const u32x4 *)&xs[(nt * 16 + (lane & 15)) * kDwPad + kq]);
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+#pragma unroll
+                for (int k0 = 0; k0 < kDwStep; k0 += 32) {
+                    if (ps + k0 >= p1) break;
+                    bf16x8 av = __builtin_bit_cast(
+                        bf16x8, *(const u32x4 *)&dzs[(mt * 16 + (lane & 15)) * kDwPad + k0 + kq]);
+                    bf16x8 bv = __builtin_bit_cast(
+                        bf16x8, *(const u32x4 *)&xs[(nt * 16 + (lane & 15)) * kDwPad + k0 + kq]);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+                }
             }
         }
         __syncthreads();
@@ -624,14 +633,27 @@ __global__ void __launch_bounds__(256) head_kernel(HeadArgs A) {
     }
 }
 // Sums head partials: loss (scaled by invCount) into *loss, head gradient into g.
-__global__ void head_reduce(const float *part, int blocks, float invCount, float *loss, float *gW, float *gB) {
-    const int k = threadIdx.x;
-    if (k >= 100) return;
+// One block per head output k (the loss, 96 weight and 3 bias gradients):
+// thread t sums the partials of blocks t, t + 256, ... in order, then a fixed
+// LDS tree adds the 256 sums.
+__global__ void __launch_bounds__(256) head_reduce(const float *part, int blocks, float invCount, float *loss,
+                                                   float *gW, float *gB) {
+    __shared__ float red[256];
+    const int t = threadIdx.x, k = blockIdx.x;
     float s = 0.0f;
-    for (int b = 0; b < blocks; ++b) s += part[(size_t)b * 100 + k];
-    if (k == 0) *loss = s * invCount;
-    else if (k <= 96) gW[k - 1] = s;
-    else gB[k - 97] = s;
+    for (int b = t; b < blocks; b += 256) s += part[(size_t)b * 100 + k];
+    red[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const float v = red[0];
+        if (k == 0) *loss = v * invCount;
+        else if (k <= 96) gW[k - 1] = v;
+        else gB[k - 97] = v;
+    }
 }
 
 // Features from raw render channels (radiance rgb, albedo rgb, normal xyz,
@@ -837,6 +859,7 @@ size_t level_pix(int N, int H, int W, int level) { return (size_t)N * (H >> leve
 size_t level_cap(size_t cap, int level) { return cap >> (2 * level); }
 
 constexpr int kDwPixPerChunk = 1024;
+static_assert(kDwPixPerChunk % kDwStep == 0, "chunks hold whole staging rounds");
 
 // Grows the workspace to hold N x H x W (H, W multiples of 16); buffers are
 // kept for smaller shapes, so alternating view sizes do not reallocate.
@@ -1043,7 +1066,7 @@ hipError_t forward_backward(rc_ctx *ctx, Net &n, int N, int H, int W, float *los
     h.invCount = 1.0f / (float)(np * 3);
     const unsigned hb = blocks_for(np);
     hipLaunchKernelGGL(head_kernel, dim3(hb), dim3(256), 0, s, h);
-    hipLaunchKernelGGL(head_reduce, dim3(1), dim3(128), 0, s, n.hpart, (int)hb, h.invCount, lossDev,
+    hipLaunchKernelGGL(head_reduce, dim3(100), dim3(256), 0, s, n.hpart, (int)hb, h.invCount, lossDev,
                        n.grad + n.pOff[9], n.grad + n.bOff[9]);
     // decoders d1 (8), d2 (7), d3 (6), d4 (5): weight gradient, input gradient,
     // then the up-sampled source's dZ.
