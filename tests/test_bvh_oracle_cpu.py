@@ -8,16 +8,13 @@ The oracle is pinned by hand-worked builds (bucket assignment, split costs, the
 partition's swap order, the axis tie rule, the leaf rules, the exception the
 Java builder raises on a thin node) and by closed-form light records (areas,
 powers and CDFs of equal lights, the skip rules)."""
-import os
-import sys
 
 import numpy as np
 import pytest
 
+from oracle import rc_oracle_bvh as ob
 from retrocam_amd.scene import Material, SceneBuilder
 
-sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
-import rc_oracle_bvh as ob  # noqa: E402
 
 f32 = np.float32
 

@@ -10,16 +10,13 @@ one float, so the comparison holds for any conforming JVM. Its exact values
 are pinned by known answers (sin and cos at float pi, double pi/2, zero) and
 against the platform libm."""
 import math
-import os
-import sys
 
 import numpy as np
 import pytest
 
+from oracle import rc_oracle_tess as ot
 from retrocam_amd.scene import Material, SceneBuilder
 
-sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
-import rc_oracle_tess as ot  # noqa: E402
 
 f32 = np.float32
 
