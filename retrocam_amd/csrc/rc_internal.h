@@ -87,6 +87,7 @@ struct Sppm {
     int *heads = nullptr;
     int *next = nullptr;
     unsigned int *counter = nullptr;
+    unsigned int *occupied = nullptr;  // hashSize bits: cells holding a photon (gather skips the rest)
 };
 
 // Wavefront path-tracing buffers (rc_wavefront.hip), sized in 256-slot tiles.

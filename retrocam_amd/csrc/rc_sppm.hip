@@ -32,6 +32,7 @@ struct TraceArgs {
     float4 *photons;
     int *heads, *next;
     unsigned int *counter;
+    unsigned int *occupied;  // one bit per cell: set when a photon is stored in it
 };
 
 // photon_trace.comp:168-251, one work-item per photon.
@@ -112,7 +113,9 @@ __global__ void __launch_bounds__(kBlock) photon_trace_kernel(TraceArgs A) {
             if (idx < (unsigned int)A.photonCount) {
                 int cx = (int)floorf(hp.x / A.radius), cy = (int)floorf(hp.y / A.radius),
                     cz = (int)floorf(hp.z / A.radius);
-                int old = atomicExch(&A.heads[cell_hash(cx, cy, cz, A.hashSize)], (int)idx);
+                const int h = cell_hash(cx, cy, cz, A.hashSize);
+                int old = atomicExch(&A.heads[h], (int)idx);
+                atomicOr(&A.occupied[h >> 5], 1u << (h & 31));
                 A.next[idx] = old;
                 float4 *rec = A.photons + 3 * (size_t)idx;
                 rec[0] = make_float4(hp.x, hp.y, hp.z, 0.0f);
@@ -131,6 +134,7 @@ struct GatherArgs {
     float radius;
     const float4 *photons;
     const int *heads, *next;
+    const unsigned int *occupied;  // cells holding at least one photon
     float4 *accum;
     double4 *accum64;
 };
@@ -191,7 +195,11 @@ __global__ void __launch_bounds__(kBlock) photon_gather_kernel(GatherArgs A) {
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                int idx = A.heads[cell_hash(cx + dx, cy + dy, cz + dz, A.hashSize)];
+                // a cell whose bit is clear has head -1: skipping it visits the same photons
+                // in the same order without fetching its head
+                const int h = cell_hash(cx + dx, cy + dy, cz + dz, A.hashSize);
+                if (!((A.occupied[h >> 5] >> (h & 31)) & 1u)) continue;
+                int idx = A.heads[h];
                 int safety = 0;
                 while (idx >= 0 && safety < 512) {
                     const float4 *ph = A.photons + 3 * (size_t)idx;
@@ -227,7 +235,8 @@ void free_sppm(Sppm &s) {
     if (s.heads) (void)hipFree(s.heads);
     if (s.next) (void)hipFree(s.next);
     if (s.counter) (void)hipFree(s.counter);
-    s.photons = nullptr; s.heads = nullptr; s.next = nullptr; s.counter = nullptr;
+    if (s.occupied) (void)hipFree(s.occupied);
+    s.photons = nullptr; s.heads = nullptr; s.next = nullptr; s.counter = nullptr; s.occupied = nullptr;
     s.capPhotons = s.capHash = 0;
 }
 
@@ -245,6 +254,7 @@ static int ensure_sppm(rc_ctx *ctx, int photons, int hashSize) {
     if (e == hipSuccess) e = hipMalloc((void **)&s.next, (size_t)cp * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&s.heads, (size_t)ch * 4);
     if (e == hipSuccess) e = hipMalloc((void **)&s.counter, 64);
+    if (e == hipSuccess) e = hipMalloc((void **)&s.occupied, (size_t)((ch + 31) / 32) * 4);
     if (e != hipSuccess) {
         free_sppm(s);
         return hip_fail(ctx, e, "hipMalloc(photon map)");
@@ -295,6 +305,9 @@ int rc_sppm_trace(rc_ctx *ctx, int32_t photonCount, int32_t hashSize) {
     // PhotonMap.clear (PhotonMap.java:84-106): heads := -1, counter := 0 (fill kernels)
     hipLaunchKernelGGL(fill_i32_kernel, dim3((hashSize + 255) / 256), dim3(256), 0, ctx->stream, s.heads, hashSize, -1);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, ctx->stream, (int *)s.counter, 1, 0);
+    const int words = (hashSize + 31) / 32;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((words + 255) / 256), dim3(256), 0, ctx->stream, (int *)s.occupied, words,
+                       0);
     s.photonCount = photonCount;
     s.hashSize = hashSize;
     if (ctx->scene.nLights > 0) {
@@ -308,6 +321,7 @@ int rc_sppm_trace(rc_ctx *ctx, int32_t photonCount, int32_t hashSize) {
         A.heads = s.heads;
         A.next = s.next;
         A.counter = s.counter;
+        A.occupied = s.occupied;
         hipLaunchKernelGGL(photon_trace_kernel, dim3((photonCount + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            ctx->stream, A);
     }
@@ -337,6 +351,7 @@ int rc_sppm_gather(rc_ctx *ctx, const rc_camera *cam) {
     A.photons = s.photons;
     A.heads = s.heads;
     A.next = s.next;
+    A.occupied = s.occupied;
     A.accum = ctx->img.accum;
     A.accum64 = ctx->img.accum64;
     if (int rc = camera_slot(ctx, cam_args(*cam), &A.cam)) return rc;
