@@ -74,22 +74,25 @@ static int capture_needs_reset(rc_ctx *ctx) {
     return set_error(ctx, RC_ERR_STATE, "a captured frame must begin with rc_reset before rendering into it");
 }
 
-__global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 __global__ void fill_temporal_kernel(rc_temporal *p, int n, rc_temporal t) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = t;
 }
 
-// Zero fill by kernel (a graph-capturable node; no memset nodes in captured frames).
-static hipError_t zero_fill(hipStream_t s, void *p, size_t bytes) {
-    const size_t n = bytes / 4;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (uint32_t *)p, n, 0u);
-    return hipGetLastError();
+// Zero fill of up to 8 buffers in one launch (a graph-capturable kernel node;
+// no memset nodes in captured frames). end[k] = words in buffers 0..k.
+struct ZeroSegs {
+    uint32_t *p[8];
+    size_t end[8];
+    int count;
+};
+
+__global__ void zero_segs_kernel(ZeroSegs z) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= z.end[z.count - 1]) return;
+    int k = 0;
+    while (i >= z.end[k]) ++k;
+    z.p[k][i - (k ? z.end[k - 1] : 0)] = 0u;
 }
 
 template <typename T>
@@ -571,11 +574,19 @@ int rc_reset(rc_ctx *ctx) {
     if (im.width > 0) {
         (void)hipSetDevice(ctx->device);
         size_t n = (size_t)im.width * (size_t)im.height;
-        size_t sizes[] = {n * 16, n * 8, n * 8, n * 4, n * 4, n * 4, n * 32};
+        const size_t sizes[] = {n * 16, n * 8, n * 8, n * 4, n * 4, n * 4, n * 32};
         void *ptrs[] = {im.accum, im.gbuf, im.galbedo, im.variance, im.primId, im.gbufId, im.accum64};
+        ZeroSegs z{};
+        size_t words = 0;
         for (int i = 0; i < 7; ++i) {
             if (!ptrs[i]) continue;
-            hipError_t e = zero_fill(ctx->stream, ptrs[i], sizes[i]);
+            words += sizes[i] / 4;
+            z.p[z.count] = (uint32_t *)ptrs[i];
+            z.end[z.count++] = words;
+        }
+        if (z.count > 0) {
+            hipLaunchKernelGGL(zero_segs_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, ctx->stream, z);
+            hipError_t e = hipGetLastError();
             if (e != hipSuccess) return hip_fail(ctx, e, "reset fill");
         }
     }
