@@ -32,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU lane-operation peak: 256 CUs x 4 SIMD16 x 16 lanes x 2.4 GHz (one lane-op per lane per cycle)
+VALU_LANE_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
 
 # Algorithmic bytes per counted event (reference data layout, per access):
 BYTE_MODEL = {
@@ -323,24 +325,22 @@ def sppm_leg(ctx, iters, log):
     return out
 
 
-def read_traffic(pt_mode, tuning):
-    """HBM bytes per sample of a full-frame path-trace launch, from the committed
-    rocprofv3 PMC summary of that kernel (FETCH_SIZE + WRITE_SIZE scaled by a copy
-    kernel of known size, divided by the launch's samples). Only when this run's
-    tuning is the profiled one (spilling variants move several times more bytes);
-    otherwise None."""
-    name = {"lanes": "pathtrace_lanes_kernel__full_frame__pmc.md",
-            "megakernel": "pathtrace_kernel__full_frame__pmc.md"}.get(pt_mode)
+def read_profile_lines(pt_mode, tuning, name_by_mode, keys, tuning_key):
+    """Per-sample figures of a full-frame path-trace launch from a committed
+    rocprofv3 summary under profiles/ (lines `key: value`), or None when the
+    file is missing or this run's lane tuning is not the profiled one (other
+    variants issue and move different amounts)."""
+    name = name_by_mode.get(pt_mode)
     if name is None:
         return None
-    path = os.path.join(ROOT, "profiles", name)
-    value, profiled = None, None
+    out, profiled = {}, None
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_spp:"):
-                value = float(line.split(":", 1)[1])
-            elif line.startswith("traffic_tuning:"):
-                profiled = {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            k, _, v = line.partition(":")
+            if k in keys:
+                out[k] = float(v)
+            elif k == tuning_key:
+                profiled = {a: int(b) for a, b in (kv.split("=") for kv in v.split())}
     except OSError:
         return None
     if pt_mode == "lanes" and tuning is not None and profiled is not None:
@@ -348,7 +348,24 @@ def read_traffic(pt_mode, tuning):
                 "tile_width": tuning["laneTileWidth"]}
         if mine != profiled:
             return None
-    return value
+    return out if len(out) == len(keys) else None
+
+
+def read_traffic(pt_mode, tuning):
+    """HBM bytes per sample (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of
+    known size, divided by the launch's samples)."""
+    r = read_profile_lines(pt_mode, tuning, {"lanes": "pathtrace_lanes_kernel__full_frame__pmc.md",
+                                             "megakernel": "pathtrace_kernel__full_frame__pmc.md"},
+                           ["traffic_bytes_per_spp"], "traffic_tuning")
+    return r["traffic_bytes_per_spp"] if r else None
+
+
+def read_valu(pt_mode, tuning):
+    """VALU lane-operations (SQ_THREAD_CYCLES_VALU) and VALU instructions
+    (SQ_INSTS_VALU) per sample, and the lane utilisation, of the lane kernel."""
+    return read_profile_lines(pt_mode, tuning, {"lanes": "pathtrace_lanes_kernel__full_frame__valu.md"},
+                              ["valu_lane_ops_per_spp", "valu_insts_per_spp", "valu_lane_utilisation"],
+                              "valu_tuning")
 
 
 # U-Net methods of the quality sweep: one pass, or the mean of 2 / 4 passes over
@@ -389,6 +406,8 @@ def main():
     ap.add_argument("--weights", default=None, help="denoiser weights (.npy; default: the committed ones)")
     ap.add_argument("--eager", action="store_true", help="time eager calls instead of replays of the captured frame")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default lane-scheduler tuning")
+    ap.add_argument("--heldout-seeds", default="901,902",
+                    help="full-frame generator seeds for the held-out quality check ('' = skip)")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -505,6 +524,40 @@ def main():
         spp, method = (args.fixed_spp or 128), (args.denoiser if args.fixed_spp else "atrous")
     quality_met = bool(quality and method in op and op[method]["spp"] == spp)
 
+    if quality and method in op and quality_met:
+        # relMSE after the VHS passes too: the same 14 passes and time inputs on the
+        # denoised frame and on the reference image
+        trace(spp)
+        use_method(method)
+        post.run_on_accum(pp_full, temporal, read=False)
+        vhs_x = r.read(N.RC_BUF_POST_HDR)[:, :3].astype(np.float64)
+        ref_rgba = np.ones((npx, 4), np.float32)
+        ref_rgba[:, :3] = ref
+        vhs_r = post.run_on_image(ref_rgba, w.camera.width, w.camera.height,
+                                  make_params(enable=VHS_PASSES), temporal).hdr[:, :3].astype(np.float64)
+        quality["post_vhs_relmse_at_operating_point"] = round(rel_mse(vhs_x, vhs_r), 6)
+        # held-out frames: the same generator at seeds no training set or weight
+        # selection used, at the headline operating point (method and spp)
+        held = []
+        for seed in [int(v) for v in args.heldout_seeds.split(",") if v.strip()]:
+            wh = workloads.full_frame(seed=seed, spp=spp)
+            SceneUploader(ctx).upload(wh.scene)
+            r.set_accum_mode(N.RC_ACCUM_FP64)
+            r.reset()
+            for _ in range(args.ref_spp // 256):
+                r.render(wh.camera, wh.params, n_samples=256)
+            a64 = r.read(N.RC_BUF_ACCUM64)
+            href = a64[:, :3] / a64[:, 3:4]
+            r.set_accum_mode(N.RC_ACCUM_FP32)
+            r.reset()
+            r.render(wh.camera, wh.params, n_samples=spp)
+            errs = {m: round(rel_mse(denoised(m), href), 6) for m in ("atrous", method)}
+            held.append({"seed": seed, "spp": spp, "relmse": errs})
+            log(f"[bench] held-out frame seed {seed} at {spp} spp: {errs}")
+        if held:
+            quality["held_out_frames"] = held
+        SceneUploader(ctx).upload(w.scene)
+
     # Counting pass: the instrumented megakernel (identical image) counts the work of
     # the reference's own walks (ordered closest hit, unordered shadow rays), which is
     # what the byte model prices; the lane scheduler does the same work or less.
@@ -592,6 +645,38 @@ def main():
     ran = {N.RC_PT_LANES: "lanes", N.RC_PT_MEGAKERNEL: "megakernel", N.RC_PT_WAVEFRONT: "wavefront"}[
         tuning["ptMode"] if tuning else pt_mode]
     traffic_spp = read_traffic(ran, tuning)
+    valu = read_valu(ran, tuning)
+    traffic = int(traffic_spp * spp) if traffic_spp else None
+    hbm = {"achieved": round(traffic / avg_kernel_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(traffic / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+           "source": "FETCH_SIZE + WRITE_SIZE per launch (profiles/pathtrace_lanes_kernel__full_frame__pmc.md)"} \
+        if traffic else None
+    kernel = {"lanes": "pathtrace_lanes_kernel (+ lanes_reduce_kernel)", "megakernel": "pathtrace_kernel",
+              "wavefront": "wavefront frame"}[ran]
+    side = {"alg_bytes_reference_layout": int(alg_bytes),
+            "alg_GBps_reference_layout": round(achieved, 1),
+            "note": "reference-layout bytes of every counted node/box/triangle/hit/material/light access, "
+                    "cache-resident re-reads priced as HBM reads: a side figure, not the bound"}
+    if valu:
+        # The kernel is bound by VALU issue under divergence: useful lane-operations
+        # (SQ_THREAD_CYCLES_VALU per sample, committed profile) over the live launch time.
+        lane_t = valu["valu_lane_ops_per_spp"] * spp / avg_kernel_s / 1e12
+        issue_t = valu["valu_insts_per_spp"] * 64 * spp / avg_kernel_s / 1e12
+        roofline = {"bound": "valu", "achieved": round(lane_t, 3), "peak": round(VALU_LANE_PEAK_T, 2),
+                    "unit": "T VALU lane-ops/s", "frac": round(lane_t / VALU_LANE_PEAK_T, 4), "traffic": traffic,
+                    "issue_frac": round(issue_t / VALU_LANE_PEAK_T, 4),
+                    "lane_utilisation": valu["valu_lane_utilisation"],
+                    "source": "SQ_THREAD_CYCLES_VALU / SQ_INSTS_VALU per sample "
+                              "(profiles/pathtrace_lanes_kernel__full_frame__valu.md) over hipEvent launch time",
+                    "hbm": hbm, "kernel": kernel, "avg_launch_ms": round(avg_kernel_s * 1e3, 3), **side}
+    elif hbm:
+        roofline = {"bound": "hbm", **hbm, "traffic": traffic, "kernel": kernel,
+                    "avg_launch_ms": round(avg_kernel_s * 1e3, 3), **side}
+    else:
+        roofline = {"bound": "valu", "achieved": None, "peak": round(VALU_LANE_PEAK_T, 2),
+                    "unit": "T VALU lane-ops/s", "frac": None, "traffic": None,
+                    "note": "this tuning has no committed counter profile", "kernel": kernel,
+                    "avg_launch_ms": round(avg_kernel_s * 1e3, 3), **side}
     out = {
         "metric": "854x480 frames/s at relMSE<=0.01 vs 4096spp (net vs a-trous); Mrays/s",
         "value": round(fps, 4),
@@ -622,17 +707,13 @@ def main():
         "trace_ms": round(float(np.mean(kms)), 3),
         "post_ms": round(post_ms, 4),
         "plain_128spp_atrous_frames_per_s": plain128,
+        "speedup_vs_plain_128spp_atrous": round(fps / plain128, 2) if plain128 else None,
+        "speedup_vs_raw_at_equal_quality": (round(fps / op["none"]["frames_per_s"], 2)
+                                            if quality_met and "none" in op else None),
         "quality": quality,
         "serving": serve,
         "sppm": sppm,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": int(traffic_spp * spp) if traffic_spp else None,
-                     "kernel": {"lanes": "pathtrace_lanes_kernel (+ lanes_reduce_kernel)",
-                                "megakernel": "pathtrace_kernel",
-                                "wavefront": "wavefront frame"}[ran],
-                     "alg_bytes_per_launch": int(alg_bytes),
-                     "avg_launch_ms": round(avg_kernel_s * 1e3, 3)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,7 +1,7 @@
 /*
  * rc_oracle.c — CPU oracle: scalar restatement of pathtrace.comp.
  *
- * TEST INFRASTRUCTURE ONLY (see rc_oracle.h). Built with -O2 -ffp-contract=off.
+ * TEST INFRASTRUCTURE ONLY (see rc_oracle.h). Built with -O3 -ffp-contract=off.
  * Paths cited below are relative to src/main/resources/shaders/.
  */
 #include "rc_oracle.h"
@@ -316,6 +316,67 @@ static void gen_ray(const rc_camera *c, int px, int py, uint32_t *seed, float fO
     *rd = normalize3(sub(fp, o));
 }
 
+/* ---- specular scattering of traceRadiance -------------------------------- */
+/* Glass (pathtrace.comp:504-523): Schlick F on r0 = ((1-ior)/(1+ior))^2 at the
+ * facing normal n, one rand picks reflection (u < F) or refraction (eta = 1/ior
+ * entering, ior leaving; total internal reflection reflects); the new origin is
+ * 2e-3 along the new direction; throughput *= albedo. geomN is the hit normal
+ * before facing, which decides entering. *Fout (may be NULL) receives F. */
+static void glass_step(float ior, v3 albedo, v3 pos, v3 geomN, v3 n, uint32_t *seed, v3 *ro, v3 *rd, v3 *thr,
+                       float *Fout) {
+    v3 d = *rd;
+    int entering = dot3(d, geomN) < 0.0f;
+    float eta = entering ? (1.0f / ior) : ior;
+    float cosI = fabsf(dot3(d, n));
+    float r0 = (1.0f - ior) / (1.0f + ior);
+    r0 = r0 * r0;
+    float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
+    if (Fout) *Fout = F;
+    if (rco_rand(seed) < F) {
+        d = reflect3(d, n);
+        *ro = add(pos, muls(d, 2e-3f));
+    } else {
+        v3 rf = refract3(d, n, eta);
+        if (dot3(rf, rf) < 0.001f) rf = reflect3(d, n);
+        d = normalize3(rf);
+        *ro = add(pos, muls(d, 2e-3f));
+    }
+    *rd = d;
+    *thr = mul(*thr, albedo);
+}
+
+/* Metal (pathtrace.comp:525-566): alpha = max(roughness, 1e-3) used as the GGX
+ * alpha, a half vector from the visible normals in the frame (T, n, B), the
+ * mirror direction about n when the reflection falls below the surface,
+ * throughput *= F * G2 / G1 with Schlick F on the albedo; new origin 2e-3 along n. */
+static void metal_step(float rough, v3 albedo, v3 pos, v3 n, uint32_t *seed, v3 *ro, v3 *rd, v3 *thr) {
+    v3 wo = muls(*rd, -1.0f);
+    float alpha = fmaxf(rough, 1e-3f);
+    v3 up = fabsf(n.y) < 0.9999f ? V(0.0f, 1.0f, 0.0f) : V(1.0f, 0.0f, 0.0f);
+    v3 T = normalize3(cross3(up, n));
+    v3 B = cross3(n, T);
+    v3 wl = V(dot3(wo, T), dot3(wo, n), dot3(wo, B));
+    float u1 = rco_rand(seed);
+    float u2 = rco_rand(seed);
+    v3 hl = sample_vndf(wl, alpha, u1, u2);
+    v3 hv = normalize3(add(add(muls(T, hl.x), muls(n, hl.y)), muls(B, hl.z)));
+    v3 mwo = muls(wo, -1.0f);
+    v3 d = reflect3(mwo, hv);
+    if (dot3(d, n) <= 0.0f) d = reflect3(mwo, n);
+    float nl = fmaxf(dot3(n, d), 1e-4f);
+    float nv = fmaxf(dot3(n, wo), 1e-4f);
+    float hv_ = fmaxf(dot3(hv, wo), 1e-4f);
+    float pw = rc_powf(1.0f - hv_, 5.0f);
+    v3 F = V(albedo.x + (1.0f - albedo.x) * pw, albedo.y + (1.0f - albedo.y) * pw,
+             albedo.z + (1.0f - albedo.z) * pw);
+    float g1 = smith_g1(nv, alpha);
+    float g2 = smith_g2(nl, nv, alpha);
+    float g = g2 / fmaxf(g1, 1e-7f);
+    *thr = mul(*thr, muls(F, g));
+    *ro = add(pos, muls(n, 2e-3f));
+    *rd = d;
+}
+
 /* ---- traceRadiance (pathtrace.comp:454-585) ----------------------------- */
 static v3 trace_radiance(const rco_scene *s, const rc_pt_params *p, v3 ro, v3 rd, uint32_t *seed,
                          v3 *firstAlbedo, int *firstTri, rco_stats *st) {
@@ -345,48 +406,10 @@ static v3 trace_radiance(const rco_scene *s, const rc_pt_params *p, v3 ro, v3 rd
             thr = divs(thr, surv);
         }
         if (trans > 0.5f) {
-            int entering = dot3(rd, h.normal) < 0.0f;
-            float eta = entering ? (1.0f / ior) : ior;
-            float cosI = fabsf(dot3(rd, n));
-            float r0 = (1.0f - ior) / (1.0f + ior);
-            r0 = r0 * r0;
-            float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
-            if (rco_rand(seed) < F) {
-                rd = reflect3(rd, n);
-                ro = add(h.pos, muls(rd, 2e-3f));
-            } else {
-                v3 rf = refract3(rd, n, eta);
-                if (dot3(rf, rf) < 0.001f) rf = reflect3(rd, n);
-                rd = normalize3(rf);
-                ro = add(h.pos, muls(rd, 2e-3f));
-            }
-            thr = mul(thr, albedo);
+            glass_step(ior, albedo, h.pos, h.normal, n, seed, &ro, &rd, &thr, NULL);
             lastDiffuse = 0;
         } else if (metallic > 0.5f) {
-            v3 wo = muls(rd, -1.0f);
-            float alpha = fmaxf(rough, 1e-3f);
-            v3 up = fabsf(n.y) < 0.9999f ? V(0.0f, 1.0f, 0.0f) : V(1.0f, 0.0f, 0.0f);
-            v3 T = normalize3(cross3(up, n));
-            v3 B = cross3(n, T);
-            v3 wl = V(dot3(wo, T), dot3(wo, n), dot3(wo, B));
-            float u1 = rco_rand(seed);
-            float u2 = rco_rand(seed);
-            v3 hl = sample_vndf(wl, alpha, u1, u2);
-            v3 hv = normalize3(add(add(muls(T, hl.x), muls(n, hl.y)), muls(B, hl.z)));
-            v3 mwo = muls(wo, -1.0f);
-            rd = reflect3(mwo, hv);
-            if (dot3(rd, n) <= 0.0f) rd = reflect3(mwo, n);
-            float nl = fmaxf(dot3(n, rd), 1e-4f);
-            float nv = fmaxf(dot3(n, wo), 1e-4f);
-            float hv_ = fmaxf(dot3(hv, wo), 1e-4f);
-            float pw = rc_powf(1.0f - hv_, 5.0f);
-            v3 F = V(albedo.x + (1.0f - albedo.x) * pw, albedo.y + (1.0f - albedo.y) * pw,
-                     albedo.z + (1.0f - albedo.z) * pw);
-            float g1 = smith_g1(nv, alpha);
-            float g2 = smith_g2(nl, nv, alpha);
-            float g = g2 / fmaxf(g1, 1e-7f);
-            thr = mul(thr, muls(F, g));
-            ro = add(h.pos, muls(n, 2e-3f));
+            metal_step(rough, albedo, h.pos, n, seed, &ro, &rd, &thr);
             lastDiffuse = 0;
         } else {
             if (p->neeEnabled != 0) rad = add(rad, mul(thr, direct_light(s, p, h.pos, n, albedo, seed, st)));
@@ -518,5 +541,50 @@ int32_t rco_fmath(int32_t fn, const float *x, const float *y, float *out, int32_
             default: return -1;
         }
     }
+    return 0;
+}
+
+/* ---- sub-step probes (for the oracle's own closed-form pins) -------------- */
+int32_t rco_gen_ray(const rc_camera *cam, int32_t px, int32_t py, uint32_t *seed, float focalOffset, float *ro,
+                    float *rd) {
+    if (!cam || !seed || !ro || !rd || cam->width <= 0 || cam->height <= 0) return -1;
+    v3 o, d;
+    gen_ray(cam, px, py, seed, focalOffset, &o, &d);
+    ro[0] = o.x; ro[1] = o.y; ro[2] = o.z;
+    rd[0] = d.x; rd[1] = d.y; rd[2] = d.z;
+    return 0;
+}
+
+void rco_sample_ngon(uint32_t *seed, float radius, int32_t blades, float *xy) {
+    sample_ngon(seed, radius, blades, &xy[0], &xy[1]);
+}
+
+void rco_sample_vndf(const float *wo, float alpha, float u1, float u2, float *h) {
+    v3 r = sample_vndf(ld3(wo), alpha, u1, u2);
+    h[0] = r.x; h[1] = r.y; h[2] = r.z;
+}
+
+float rco_smith_g1(float nv, float alpha) { return smith_g1(nv, alpha); }
+float rco_smith_g2(float nl, float nv, float alpha) { return smith_g2(nl, nv, alpha); }
+
+void rco_sample_cosine(float u1, float u2, const float *n, float *dir) {
+    v3 r = sample_cosine(u1, u2, ld3(n));
+    dir[0] = r.x; dir[1] = r.y; dir[2] = r.z;
+}
+
+int32_t rco_scatter(const float *mat, const float *rd, const float *geomNormal, const float *pos, uint32_t *seed,
+                    float *roOut, float *rdOut, float *thrOut, float *fresnelOut) {
+    if (!mat || !rd || !geomNormal || !pos || !seed || !roOut || !rdOut || !thrOut) return -1;
+    v3 d = ld3(rd), gn = ld3(geomNormal), P = ld3(pos);
+    v3 n = dot3(d, gn) < 0.0f ? gn : muls(gn, -1.0f);
+    v3 albedo = ld3(mat), thr = V(1.0f, 1.0f, 1.0f), o = V(0, 0, 0);
+    float F = -1.0f;
+    if (mat[10] > 0.5f) glass_step(mat[9], albedo, P, gn, n, seed, &o, &d, &thr, &F);
+    else if (mat[3] > 0.5f) metal_step(mat[8], albedo, P, n, seed, &o, &d, &thr);
+    else return -1;
+    roOut[0] = o.x; roOut[1] = o.y; roOut[2] = o.z;
+    rdOut[0] = d.x; rdOut[1] = d.y; rdOut[2] = d.z;
+    thrOut[0] = thr.x; thrOut[1] = thr.y; thrOut[2] = thr.z;
+    if (fresnelOut) *fresnelOut = F;
     return 0;
 }

@@ -124,6 +124,30 @@ uint32_t rco_hash(int32_t kind, uint32_t a, uint32_t b);
  * 5 binary32 -> binary16 (RNE) -> binary32. Returns 0, or -1 for a bad fn. */
 int32_t rco_fmath(int32_t fn, const float *x, const float *y, float *out, int32_t n);
 
+/* ---- sub-steps of pathtrace.comp, for the oracle's own closed-form pins ---- */
+/* generateRay (pathtrace.comp:401-439) for pixel (px, py) from the seed stream
+ * *seed (advanced as the shader advances it), focal plane offset focalOffset
+ * (the LCA channel offset). Returns 0 or -1. */
+int32_t rco_gen_ray(const rc_camera *cam, int32_t px, int32_t py, uint32_t *seed, float focalOffset, float *ro,
+                    float *rd);
+/* sampleNGon (pathtrace.comp:310-321): lens point xy[2] of radius/blades. */
+void rco_sample_ngon(uint32_t *seed, float radius, int32_t blades, float *xy);
+/* sampleGGXVNDF (pathtrace.comp:252-276) in the local frame with y = normal. */
+void rco_sample_vndf(const float *wo, float alpha, float u1, float u2, float *h);
+/* smithG1GGX / smithG2GGX (pathtrace.comp:278-292). */
+float rco_smith_g1(float nv, float alpha);
+float rco_smith_g2(float nl, float nv, float alpha);
+/* sampleCosineHemisphere about n (pathtrace.comp:296-306). */
+void rco_sample_cosine(float u1, float u2, const float *n, float *dir);
+/* The specular scattering step of traceRadiance at a hit (glass when
+ * mat[10] > 0.5, else metal when mat[3] > 0.5; pathtrace.comp:504-566): ray
+ * direction rd, the hit's normal before facing, hit position, one material
+ * record (16 floats) and the seed stream. Writes the new origin, direction and
+ * throughput factor, and for glass the Fresnel term (-1 for metal). Returns 0,
+ * or -1 for a diffuse material. */
+int32_t rco_scatter(const float *mat, const float *rd, const float *geomNormal, const float *pos, uint32_t *seed,
+                    float *roOut, float *rdOut, float *thrOut, float *fresnelOut);
+
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int32_t rco_num_threads(void);
 

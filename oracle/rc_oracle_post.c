@@ -578,7 +578,10 @@ uint32_t rco_hash(int32_t kind, uint32_t a, uint32_t b) {
 }
 
 /* ---- driver ---------------------------------------------------------------- */
+/* Every pass is a pure function of its input image per output pixel, so rows
+ * run in parallel without changing any value. */
 static void run_pass(int pass, img s, float *dst, const rc_post_params *p, const rc_temporal *t, const gbufs *g) {
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < s.h; ++y) {
         for (int x = 0; x < s.w; ++x) {
             c3 o;

@@ -5,6 +5,7 @@ CPU-baseline leg of bench.py, as the checker. The product never imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -50,43 +51,76 @@ def build() -> str:
 _lib = None
 
 
+def load(path: str) -> C.CDLL:
+    """Loads an oracle library built from these sources (the default build, or a
+    test's variant build) and declares its entry points."""
+    from retrocam_amd._native import RcCamera, RcPostParams, RcPtParams, RcTemporal
+    L = C.CDLL(path)
+    U32P = C.POINTER(C.c_uint32)
+    L.rco_pcg.restype = C.c_uint32
+    L.rco_pcg.argtypes = [U32P]
+    L.rco_init_seed.restype = C.c_uint32
+    L.rco_init_seed.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+    L.rco_rand.restype = C.c_float
+    L.rco_rand.argtypes = [U32P]
+    L.rco_closest_hit.restype = C.c_int32
+    L.rco_closest_hit.argtypes = [C.POINTER(RcoScene), _FP, _FP, _FP, _FP]
+    L.rco_brute_closest.restype = C.c_int32
+    L.rco_brute_closest.argtypes = [C.POINTER(RcoScene), _FP, _FP, _FP]
+    L.rco_shadow_ray.restype = C.c_int32
+    L.rco_shadow_ray.argtypes = [C.POINTER(RcoScene), _FP, _FP, C.c_float]
+    L.rco_render_pass.restype = C.c_int32
+    L.rco_render_pass.argtypes = [C.POINTER(RcoScene), C.POINTER(RcCamera), C.POINTER(RcPtParams),
+                                  C.c_int32, C.c_int32, C.POINTER(RcoFrame), C.POINTER(RcoStats)]
+    L.rco_post_chain.restype = C.c_int32
+    L.rco_post_chain.argtypes = [_FP, _VP, _VP, _FP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
+                                 C.POINTER(RcTemporal), _FP, _VP]
+    L.rco_post_pass.restype = C.c_int32
+    L.rco_post_pass.argtypes = [C.c_int32, _FP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
+                                C.POINTER(RcTemporal), _VP, _VP, _FP, C.c_int32]
+    L.rco_hash.restype = C.c_uint32
+    L.rco_hash.argtypes = [C.c_int32, C.c_uint32, C.c_uint32]
+    L.rco_fmath.restype = C.c_int32
+    L.rco_fmath.argtypes = [C.c_int32, _FP, _FP, _FP, C.c_int32]
+    L.rco_num_threads.restype = C.c_int32
+    L.rco_num_threads.argtypes = []
+    L.rco_gen_ray.restype = C.c_int32
+    L.rco_gen_ray.argtypes = [C.POINTER(RcCamera), C.c_int32, C.c_int32, U32P, C.c_float, _FP, _FP]
+    L.rco_sample_ngon.restype = None
+    L.rco_sample_ngon.argtypes = [U32P, C.c_float, C.c_int32, _FP]
+    L.rco_sample_vndf.restype = None
+    L.rco_sample_vndf.argtypes = [_FP, C.c_float, C.c_float, C.c_float, _FP]
+    L.rco_smith_g1.restype = C.c_float
+    L.rco_smith_g1.argtypes = [C.c_float, C.c_float]
+    L.rco_smith_g2.restype = C.c_float
+    L.rco_smith_g2.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.rco_sample_cosine.restype = None
+    L.rco_sample_cosine.argtypes = [C.c_float, C.c_float, _FP, _FP]
+    L.rco_scatter.restype = C.c_int32
+    L.rco_scatter.argtypes = [_FP, _FP, _FP, _FP, U32P, _FP, _FP, _FP, _FP]
+    return L
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        from retrocam_amd._native import RcCamera, RcPtParams
-        L = C.CDLL(LIB_PATH)
-        L.rco_pcg.restype = C.c_uint32
-        L.rco_pcg.argtypes = [C.POINTER(C.c_uint32)]
-        L.rco_init_seed.restype = C.c_uint32
-        L.rco_init_seed.argtypes = [C.c_int32, C.c_int32, C.c_int32]
-        L.rco_rand.restype = C.c_float
-        L.rco_rand.argtypes = [C.POINTER(C.c_uint32)]
-        L.rco_closest_hit.restype = C.c_int32
-        L.rco_closest_hit.argtypes = [C.POINTER(RcoScene), _FP, _FP, _FP, _FP]
-        L.rco_brute_closest.restype = C.c_int32
-        L.rco_brute_closest.argtypes = [C.POINTER(RcoScene), _FP, _FP, _FP]
-        L.rco_shadow_ray.restype = C.c_int32
-        L.rco_shadow_ray.argtypes = [C.POINTER(RcoScene), _FP, _FP, C.c_float]
-        L.rco_render_pass.restype = C.c_int32
-        L.rco_render_pass.argtypes = [C.POINTER(RcoScene), C.POINTER(RcCamera), C.POINTER(RcPtParams),
-                                      C.c_int32, C.c_int32, C.POINTER(RcoFrame), C.POINTER(RcoStats)]
-        from retrocam_amd._native import RcPostParams, RcTemporal
-        L.rco_post_chain.restype = C.c_int32
-        L.rco_post_chain.argtypes = [_FP, _VP, _VP, _FP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
-                                     C.POINTER(RcTemporal), _FP, _VP]
-        L.rco_post_pass.restype = C.c_int32
-        L.rco_post_pass.argtypes = [C.c_int32, _FP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
-                                    C.POINTER(RcTemporal), _VP, _VP, _FP, C.c_int32]
-        L.rco_hash.restype = C.c_uint32
-        L.rco_hash.argtypes = [C.c_int32, C.c_uint32, C.c_uint32]
-        L.rco_fmath.restype = C.c_int32
-        L.rco_fmath.argtypes = [C.c_int32, _FP, _FP, _FP, C.c_int32]
-        L.rco_num_threads.restype = C.c_int32
-        L.rco_num_threads.argtypes = []
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(library: C.CDLL):
+    """Routes every call of this module through `library` (a variant build of the
+    oracle) for the duration of the block."""
+    global _lib
+    saved = lib()
+    _lib = library
+    try:
+        yield library
+    finally:
+        _lib = saved
 
 
 def _fp(a):
@@ -212,3 +246,71 @@ def shadow_ray(scene: OracleScene, ro, rd, max_dist):
     ro = np.asarray(ro, np.float32)
     rd = np.asarray(rd, np.float32)
     return bool(lib().rco_shadow_ray(C.byref(scene.c), _fp(ro), _fp(rd), float(max_dist)))
+
+
+# ---- sub-step probes of pathtrace.comp (for the oracle's own pins) ----------
+def _u32(v: int):
+    return C.c_uint32(int(v) & 0xFFFFFFFF)
+
+
+def rand_stream(seed: int, n: int):
+    """The next n rand() values of a seed stream, and the state after them."""
+    s = _u32(seed)
+    out = [float(lib().rco_rand(C.byref(s))) for _ in range(n)]
+    return out, int(s.value)
+
+
+def gen_ray(cam_c, px: int, py: int, seed: int, focal_offset: float = 0.0):
+    """generateRay: (origin, direction, seed state after the ray)."""
+    s = _u32(seed)
+    ro = np.zeros(3, np.float32)
+    rd = np.zeros(3, np.float32)
+    if lib().rco_gen_ray(C.byref(cam_c), int(px), int(py), C.byref(s), float(focal_offset), _fp(ro), _fp(rd)) != 0:
+        raise ValueError("rco_gen_ray rejected its arguments")
+    return ro, rd, int(s.value)
+
+
+def sample_ngon(seed: int, radius: float, blades: int):
+    s = _u32(seed)
+    xy = np.zeros(2, np.float32)
+    lib().rco_sample_ngon(C.byref(s), float(radius), int(blades), _fp(xy))
+    return xy, int(s.value)
+
+
+def sample_vndf(wo, alpha: float, u1: float, u2: float):
+    wo = np.ascontiguousarray(wo, np.float32)
+    h = np.zeros(3, np.float32)
+    lib().rco_sample_vndf(_fp(wo), float(alpha), float(u1), float(u2), _fp(h))
+    return h
+
+
+def smith_g1(nv: float, alpha: float) -> float:
+    return float(lib().rco_smith_g1(float(nv), float(alpha)))
+
+
+def smith_g2(nl: float, nv: float, alpha: float) -> float:
+    return float(lib().rco_smith_g2(float(nl), float(nv), float(alpha)))
+
+
+def sample_cosine(u1: float, u2: float, n):
+    n = np.ascontiguousarray(n, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().rco_sample_cosine(float(u1), float(u2), _fp(n), _fp(d))
+    return d
+
+
+def scatter(mat, rd, geom_normal, pos, seed: int):
+    """Specular scattering step: (origin, direction, throughput factor, Fresnel
+    term or -1, seed state after)."""
+    mat = np.ascontiguousarray(mat, np.float32)
+    rd = np.ascontiguousarray(rd, np.float32)
+    gn = np.ascontiguousarray(geom_normal, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    s = _u32(seed)
+    ro = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    thr = np.zeros(3, np.float32)
+    F = C.c_float()
+    if lib().rco_scatter(_fp(mat), _fp(rd), _fp(gn), _fp(pos), C.byref(s), _fp(ro), _fp(d), _fp(thr), C.byref(F)) != 0:
+        raise ValueError("rco_scatter: not a specular material")
+    return ro, d, thr, float(F.value), int(s.value)
