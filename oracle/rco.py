@@ -98,6 +98,19 @@ def load(path: str) -> C.CDLL:
     L.rco_sample_cosine.argtypes = [C.c_float, C.c_float, _FP, _FP]
     L.rco_scatter.restype = C.c_int32
     L.rco_scatter.argtypes = [_FP, _FP, _FP, _FP, U32P, _FP, _FP, _FP, _FP]
+    I32P = C.POINTER(C.c_int32)
+    L.rco_photon_seed.restype = C.c_uint32
+    L.rco_photon_seed.argtypes = [C.c_uint32, C.c_int32]
+    L.rco_cell_hash.restype = C.c_uint32
+    L.rco_cell_hash.argtypes = [C.c_int32] * 4
+    L.rco_photon_trace.restype = C.c_int32
+    L.rco_photon_trace.argtypes = [C.POINTER(RcoScene), C.c_int32, C.c_float, C.c_int32, C.c_int32, _FP, I32P, I32P,
+                                   I32P]
+    L.rco_photon_gather.restype = C.c_int32
+    L.rco_photon_gather.argtypes = [C.POINTER(RcoScene), C.POINTER(RcCamera), C.c_float, C.c_int32, _FP, I32P, I32P,
+                                    _FP, C.c_void_p]
+    L.rco_sppm_next_radius.restype = C.c_float
+    L.rco_sppm_next_radius.argtypes = [C.c_float, C.c_int32, C.c_float]
     return L
 
 

@@ -77,17 +77,7 @@ def _trace(sc, photons=20000, radius=0.05, seed=6271, hash_size=1 << 16):
 
 
 def _bind():
-    L = rco.lib()
-    L.rco_photon_trace.restype = C.c_int32
-    L.rco_photon_trace.argtypes = [C.POINTER(rco.RcoScene), C.c_int32, C.c_float, C.c_int32, C.c_int32, rco._FP,
-                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
-    from retrocam_amd._native import RcCamera
-    L.rco_photon_gather.restype = C.c_int32
-    L.rco_photon_gather.argtypes = [C.POINTER(rco.RcoScene), C.POINTER(RcCamera), C.c_float, C.c_int32, rco._FP,
-                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32), rco._FP, C.c_void_p]
-    L.rco_cell_hash.restype = C.c_uint32
-    L.rco_cell_hash.argtypes = [C.c_int32] * 4
-    return L
+    return rco.lib()  # the SPPM entry points are declared by rco.load
 
 
 def check_hash_structure(ph, heads, nxt, stored, radius, hash_size):
