@@ -278,6 +278,20 @@ int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t);
 int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
                   const rc_temporal *t);
 
+/* No counterpart in the reference: rc_post_image for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Device pointers in and out: rgba is a w x h
+ * RGBA32F image (rows bottom-up), hdrOut (may be NULL) receives the chain output
+ * (RGBA32F) and rgba8Out the display encode (RGBA8); the chain is
+ * rc_post_image's (N = 1, exposure 1, no a-trous), enqueued on the context's
+ * stream (rc_sync waits for it). It allocates nothing: its two intermediate
+ * images live in a library-wide device array of 1280 x 720 pixels each per GPU
+ * (29.5 MB, held until the process ends); larger images return RC_ERR_LIMIT.
+ * Every call on a GPU uses that GPU's one array: calls that overlap on one GPU
+ * (other contexts' streams, other threads) race. Not allowed while capturing. */
+int rc_post_image_noalloc(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
+                          const rc_temporal *t, float *hdrOut, uint8_t *rgba8Out);
+
 /* ---- frame export ------------------------------------------------------- */
 /* FrameExporter.exportFrame (src/main/java/com/retrocam/export/FrameExporter.java:37-65):
  * the display image of the last rc_post (RGBA8, ACES + sRGB) with its rows flipped to

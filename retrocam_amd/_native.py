@@ -160,6 +160,8 @@ HIP_SYMBOLS = {
     "rc_post_defaults": (None, [C.POINTER(RcPostParams)]),
     "rc_post": (C.c_int, [_VP, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
     "rc_post_image": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
+    "rc_post_image_noalloc": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
+                                        C.POINTER(RcTemporal), _VP, _VP]),
     "rc_upload_scene_slot": (C.c_int, [_VP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP,
                                        C.c_int32, C.c_float]),
     "rc_views_render": (C.c_int, [_VP, C.POINTER(RcView), C.c_int32, C.c_int32]),

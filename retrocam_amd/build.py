@@ -33,7 +33,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["rc_capi.hip", "rc_pathtrace.hip", "rc_pathtrace_lanes.hip", "rc_wavefront.hip", "rc_debug.hip", "rc_post.hip",
-               "rc_sppm.hip", "rc_net.hip", "rc_batch.hip", "rc_tune.hip", "rc_export.hip"]
+               "rc_sppm.hip", "rc_net.hip", "rc_batch.hip", "rc_tune.hip", "rc_export.hip", "rc_noalloc.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -296,6 +296,12 @@ int rc_create(int device, rc_ctx **out) {
         delete ctx;
         return set_error(nullptr, RC_ERR_HIP, "hipStreamCreate failed");
     }
+    if (int rc = noalloc_setup(ctx, device)) {  // rc_post_image_noalloc's scratch address on this device
+        std::string m = ctx->err;
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_error(nullptr, rc, m);
+    }
     void *c = nullptr;
     e = hipMalloc(&c, 64 * sizeof(unsigned long long));
     if (e != hipSuccess) {

@@ -230,6 +230,10 @@ bool net_ready(const rc_ctx *ctx);
 int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo,
                     const float *variance, int total, float4 *dst, int w, int h);
 
+// Looks up rc_post_image_noalloc's scratch address on `device` (current) once
+// per device (rc_noalloc.hip). Returns an RC_* code.
+int noalloc_setup(rc_ctx *ctx, int device);
+
 // Post chain on explicit buffers (rc_capi.hip).
 int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf);
 // Scene of a slot (0 = rc_upload_scene's), or null when out of range.
