@@ -181,6 +181,14 @@ enum {
                                fallback when no network is loaded                */
 };
 
+/* Organisation of the post chain (rc_set_post_mode); both give identical images. */
+enum {
+    RC_POST_FUSED = 0,      /* default: the chain in a few full-frame sweeps, each
+                               running consecutive passes over a tile in LDS   */
+    RC_POST_PER_PASS = 1    /* one full-frame launch per pass, as the reference's
+                               FBO chain draws them                             */
+};
+
 typedef struct rc_ctx rc_ctx;
 
 /* Scene slots for batched views (slot 0 is the scene of rc_upload_scene). */
@@ -291,6 +299,13 @@ int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height,
  * (other contexts' streams, other threads) race. Not allowed while capturing. */
 int rc_post_image_noalloc(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,
                           const rc_temporal *t, float *hdrOut, uint8_t *rgba8Out);
+
+/* No counterpart in the reference: RC_POST_FUSED (default) or RC_POST_PER_PASS
+ * for later post chains (rc_post, rc_post_image, rc_views_post, ...); the images
+ * are bit-identical. *launches (may be NULL) receives the kernel launches of the
+ * last post chain, the denoiser network's own excluded. */
+int rc_set_post_mode(rc_ctx *ctx, int32_t mode);
+int rc_post_launches(const rc_ctx *ctx, int32_t *launches);
 
 /* ---- frame export ------------------------------------------------------- */
 /* FrameExporter.exportFrame (src/main/java/com/retrocam/export/FrameExporter.java:37-65):

@@ -42,6 +42,8 @@ RC_PT_LANES = 2
 RC_LANE_VARIANTS = 11
 
 RC_DENOISE_ATROUS = 0
+RC_POST_FUSED = 0
+RC_POST_PER_PASS = 1
 RC_DENOISE_NET = 1
 
 F3 = C.c_float * 3
@@ -160,6 +162,8 @@ HIP_SYMBOLS = {
     "rc_post_defaults": (None, [C.POINTER(RcPostParams)]),
     "rc_post": (C.c_int, [_VP, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
     "rc_post_image": (C.c_int, [_VP, _FP, C.c_int32, C.c_int32, C.POINTER(RcPostParams), C.POINTER(RcTemporal)]),
+    "rc_set_post_mode": (C.c_int, [_VP, C.c_int32]),
+    "rc_post_launches": (C.c_int, [_VP, C.POINTER(C.c_int32)]),
     "rc_post_image_noalloc": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(RcPostParams),
                                         C.POINTER(RcTemporal), _VP, _VP]),
     "rc_upload_scene_slot": (C.c_int, [_VP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP, C.c_int32, _FP,

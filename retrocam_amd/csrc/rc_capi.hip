@@ -141,11 +141,11 @@ static int ensure_post(rc_ctx *ctx, int w, int h) {
 
 // PostProcessStack.runChain (PostProcessStack.java:140-280) from the p00 input
 // io.src (accumulation sum, or an image with total = 1), then the display pass.
-int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf) {
+static int run_chain_per_pass(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t,
+                              bool withGbuf, const rc_temporal *dt) {
     const int w = io.w, h = io.h;
     float4 *a = io.a, *b = io.b;
-    const rc_temporal *dt = temporal_slot(ctx, t);
-    if (ctx->capturing && !dt) return set_error(ctx, RC_ERR_LIMIT, "captured frame: too many post chains");
+    int nl = 1;
     hipError_t e = launch_post_pass(ctx->stream, 0, io.src, a, w, h, p, t, io.gbuf, io.galbedo, io.variance,
                                     io.total, 0, nullptr, dt);
     if (e != hipSuccess) return hip_fail(ctx, e, "post p00 launch");
@@ -161,6 +161,7 @@ int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_tempor
                                  1 << i, nullptr, dt);
             if (e != hipSuccess) return hip_fail(ctx, e, "post a-trous launch");
             float4 *tmp = a; a = b; b = tmp;
+            ++nl;
         }
     }
     for (int k = 1; k <= 16; ++k) {
@@ -168,10 +169,60 @@ int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_tempor
         e = launch_post_pass(ctx->stream, k, a, b, w, h, p, t, nullptr, nullptr, nullptr, io.total, 0, nullptr, dt);
         if (e != hipSuccess) return hip_fail(ctx, e, "post pass launch");
         float4 *tmp = a; a = b; b = tmp;
+        ++nl;
     }
     e = launch_post_pass(ctx->stream, 200, a, nullptr, w, h, p, t, nullptr, nullptr, nullptr, 1, 0, io.display, dt);
     if (e != hipSuccess) return hip_fail(ctx, e, "display launch");
     io.out = a;
+    ctx->postLaunches = nl + 1;
+    return RC_OK;
+}
+
+// The same chain as fused sweeps (rc_post.hip launch_post_sweeps): p00 is read
+// on the fly by the first sweep; an a-trous pass starts a sweep of its own, the
+// last one carrying the row-local passes that follow it; with the network, p00
+// is written once for the network's input and the VHS passes start from its
+// output.
+int run_chain(rc_ctx *ctx, ChainIO &io, const rc_post_params &p, const rc_temporal &t, bool withGbuf) {
+    const rc_temporal *dt = temporal_slot(ctx, t);
+    if (ctx->capturing && !dt) return set_error(ctx, RC_ERR_LIMIT, "captured frame: too many post chains");
+    if (ctx->postMode == RC_POST_PER_PASS) return run_chain_per_pass(ctx, io, p, t, withGbuf, dt);
+    const int w = io.w, h = io.h;
+    int ops[24], steps[24], n = 0, nl = 0, l = 0;
+    SweepSource src;
+    src.p = io.src;
+    src.p00 = true;
+    hipError_t e;
+    const bool exceeded = p.atrousMaxSpp > 0 && io.total > p.atrousMaxSpp;
+    if (p.atrousEnabled && withGbuf && !exceeded && ctx->denoiser == RC_DENOISE_NET && net_ready(ctx)) {
+        const float4 *p00out = nullptr;
+        e = launch_post_sweeps(ctx->stream, src, nullptr, nullptr, 0, w, h, io.a, io.b, nullptr, p, t, dt, io.gbuf,
+                               io.galbedo, io.variance, io.total, &p00out, &l);
+        if (e != hipSuccess) return hip_fail(ctx, e, "post p00 sweep");
+        nl += l;
+        float4 *netOut = (p00out == io.a) ? io.b : io.a;
+        int rc = run_net_denoise(ctx, p00out, io.gbuf, io.galbedo, io.variance, io.total, netOut, w, h);
+        if (rc) return rc;
+        src.p = netOut;
+        src.p00 = false;
+    } else if (p.atrousEnabled && withGbuf && !exceeded) {
+        int np = p.atrousIterations < 1 ? 1 : (p.atrousIterations > 5 ? 5 : p.atrousIterations);
+        for (int i = 0; i < np; ++i) {  // AtrousDenoiser.denoise (AtrousDenoiser.java:60-125)
+            ops[n] = 100;
+            steps[n++] = 1 << i;
+        }
+    }
+    for (int k = 1; k <= 16; ++k) {
+        if (k == 5 || k == 6 || !p.enable[k]) continue;  // p05/p06 pass through (PostProcessStack.java:184)
+        ops[n] = k;
+        steps[n++] = 0;
+    }
+    const float4 *out = nullptr;
+    e = launch_post_sweeps(ctx->stream, src, ops, steps, n, w, h, io.a, io.b, io.display, p, t, dt, io.gbuf,
+                           io.galbedo, io.variance, io.total, &out, &l);
+    if (e != hipSuccess) return hip_fail(ctx, e, "post sweep launch");
+    io.out = out;
+    ctx->postLaunches = nl + l;
     return RC_OK;
 }
 
@@ -757,6 +808,20 @@ int rc_post(rc_ctx *ctx, const rc_post_params *p, const rc_temporal *t) {
     (void)hipSetDevice(ctx->device);
     if ((rc = ensure_post(ctx, ctx->img.width, ctx->img.height))) return rc;
     return run_post_chain(ctx, ctx->img.accum, ctx->img.width, ctx->img.height, *p, *t, ctx->totalSamples, true);
+}
+
+int rc_set_post_mode(rc_ctx *ctx, int32_t mode) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (mode != RC_POST_FUSED && mode != RC_POST_PER_PASS) return set_error(ctx, RC_ERR_INVALID_ARG, "bad post mode");
+    if (int rc = capture_block(ctx, "rc_set_post_mode")) return rc;
+    ctx->postMode = mode;
+    return RC_OK;
+}
+
+int rc_post_launches(const rc_ctx *ctx, int32_t *launches) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (launches) *launches = ctx->postLaunches;
+    return RC_OK;
 }
 
 int rc_post_image(rc_ctx *ctx, const float *rgba, int32_t width, int32_t height, const rc_post_params *p,

@@ -155,6 +155,8 @@ struct rc_ctx {
     rc::Net *net = nullptr;
     int32_t denoiser = RC_DENOISE_ATROUS;
     int32_t netFlips = 1;  // network passes per denoise (rc_set_net_flips)
+    int32_t postMode = RC_POST_FUSED;  // post chain as fused sweeps or one launch per pass
+    int32_t postLaunches = 0;          // launches of the last post chain (excluding the denoiser network)
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)
     int countersEnabled = 0;
     // captured frame (rc_frame_*): the graph, its instance, and the temporal
@@ -243,6 +245,24 @@ void free_views(rc_ctx *ctx);
 // Sample counts of the current views (captured frames keep and restore them).
 std::vector<int> views_totals(const rc_ctx *ctx);
 void views_set_totals(rc_ctx *ctx, const std::vector<int> &totals);
+
+// The first input of a fused post chain: a float4 image, read either as it is
+// or as the p00 output of an accumulation (sum / max(total, 1) * exposure).
+struct SweepSource {
+    const float4 *p = nullptr;
+    bool p00 = false;
+};
+// Runs the chain ops[0..nOps) (pass ids: 100 = a-trous with steps[i], 1..16 =
+// VHS pN; nOps = 0: the source copied) over a w x h image as fused sweeps,
+// ping-ponging between bufA and bufB (neither may be src.p unless src is the
+// previous output), with the display encode of the result into rgba8 (may be
+// null). *outFinal receives the buffer holding the chain output, *launches the
+// number of sweeps (rc_post.hip).
+hipError_t launch_post_sweeps(hipStream_t stream, const SweepSource &src, const int *ops, const int *steps, int nOps,
+                              int w, int h, float4 *bufA, float4 *bufB, uchar4 *rgba8, const rc_post_params &p,
+                              const rc_temporal &t, const rc_temporal *dt, const ushort4 *gbuf,
+                              const ushort4 *galbedo, const float *variance, int total, const float4 **outFinal,
+                              int *launches);
 
 // One post-chain pass (rc_post.hip): pass 0 = p00, 1..16 = VHS pN, 100 = a-trous
 // with the given step, 200 = display encode into rgba8.
