@@ -918,7 +918,7 @@ static float pass_cost(int pass, const rc_post_params &p) {
         case P_ATROUS: return 60.0f;
         case 1: return 3.0f;
         case 2: return 1.0f + 0.5f * (float)p02_radius(p);
-        case 8: return 2.0f + 0.3f * (float)std::min(std::max(p.ccdSmearLength, 0), 150);
+        case 8: return 2.0f + 0.6f * (float)std::min(std::max(p.ccdSmearLength, 0), 150);
         case 9: return 6.0f;
         case 12: return 3.0f;
         case 15: return 5.0f;
