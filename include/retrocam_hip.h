@@ -382,6 +382,17 @@ int rc_set_denoiser(rc_ctx *ctx, int32_t mode);
  * for more network time). Applies to rc_post and rc_views_post. */
 int rc_set_net_flips(rc_ctx *ctx, int32_t passes);
 
+/* Convolution kernels of the network's forward pass (no counterpart in the
+ * reference); every choice computes each output with the same operands in the
+ * same order, so the network's outputs are bit-identical:
+ *   RC_NET_KERNELS_TILED  (default) the three finest levels stage input tiles in
+ *                         LDS, small levels split output channels over waves;
+ *   RC_NET_KERNELS_DIRECT every level reads its operands per tap from L1/L2,
+ *                         channel groups as in the default;
+ *   RC_NET_KERNELS_SINGLE_GROUP  as DIRECT, one channel group per wave. */
+enum { RC_NET_KERNELS_TILED = 0, RC_NET_KERNELS_DIRECT = 1, RC_NET_KERNELS_SINGLE_GROUP = 2 };
+int rc_set_net_kernels(rc_ctx *ctx, int32_t mode);
+
 /* ---- captured frames (no counterpart in the reference) ------------------- */
 /* RenderPipeline.accumulateFrame (src/main/java/com/retrocam/export/RenderPipeline.java:209-239)
  * issues the same chain of dispatches every frame: reset, spp x {photon trace,

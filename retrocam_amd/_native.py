@@ -44,6 +44,9 @@ RC_LANE_VARIANTS = 11
 RC_DENOISE_ATROUS = 0
 RC_POST_FUSED = 0
 RC_POST_PER_PASS = 1
+RC_NET_KERNELS_TILED = 0
+RC_NET_KERNELS_DIRECT = 1
+RC_NET_KERNELS_SINGLE_GROUP = 2
 RC_DENOISE_NET = 1
 
 F3 = C.c_float * 3
@@ -191,6 +194,7 @@ HIP_SYMBOLS = {
     "rc_frame_release": (C.c_int, [_VP]),
     "rc_frame_set_camera": (C.c_int, [_VP, C.POINTER(RcCamera)]),
     "rc_set_net_flips": (C.c_int, [_VP, C.c_int32]),
+    "rc_set_net_kernels": (C.c_int, [_VP, C.c_int32]),
     "rc_get_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_set_tuning": (C.c_int, [_VP, C.POINTER(RcTuning)]),
     "rc_autotune": (C.c_int, [_VP, C.POINTER(RcCamera), C.POINTER(RcPtParams), C.c_int32, C.POINTER(RcTuning),

@@ -155,6 +155,7 @@ struct rc_ctx {
     rc::Net *net = nullptr;
     int32_t denoiser = RC_DENOISE_ATROUS;
     int32_t netFlips = 1;  // network passes per denoise (rc_set_net_flips)
+    int32_t netKernels = RC_NET_KERNELS_TILED;  // forward convolution kernels (rc_set_net_kernels)
     int32_t postMode = RC_POST_FUSED;  // post chain as fused sweeps or one launch per pass
     int32_t postLaunches = 0;          // launches of the last post chain (excluding the denoiser network)
     unsigned long long *counters = nullptr;  // device work counters (bench instrumentation)

@@ -35,9 +35,10 @@
 //                       read in place — no concat or upsample buffers. Also the
 //                       input gradient: dX = W' * dZ with W' the tap-flipped,
 //                       transposed weights.
-//   conv3x3_fwd_lds<NT,MT>  the forward convolution of the three finest levels
-//                       with each block's input tile staged in LDS (same K
-//                       order and operands: bit-identical outputs).
+//   conv3x3_tile<CIN,COUT,TH,G>  the forward convolution of the three finest
+//                       levels: weight fragments in registers, each block's
+//                       input tile staged in LDS (same K order and operands:
+//                       bit-identical outputs).
 //   conv3x3_dw          dW[co][tap][ci] = sum_p dZ[p][co] X[p+tap][ci]: a GEMM
 //                       with K = pixels, operands transposed through LDS,
 //                       per-chunk partials reduced in a fixed order
@@ -114,6 +115,7 @@ struct ConvArgs {
     int cout, relu;
     uint16_t *outB;     // bf16 output, or
     float *outF;        // f32 output
+    const uint16_t *zero;  // >= 8 zero bf16 (conv3x3_tile's padding source)
 };
 
 template <int NT, int MT, int G>
@@ -196,96 +198,195 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
     }
 }
 
-// The same convolution with the input staged in LDS: a block computes a
-// (4 MT) x 16 tile of output pixels of one image (wave w: rows MT w .. MT w +
-// MT - 1, 16 pixels each), after loading the tile's (4 MT + 2) x 18 input halo,
-// every input channel, once (zero outside the image; the a source upsampled in
-// place). The K order per output (tap, then 32-channel chunk) and the operands
-// are those of conv3x3_fwd, so the outputs are bit-identical; every input
-// pixel is read from HBM/L2 once per tile instead of once per tap, and each
-// weight fragment feeds MT MFMAs. LDS rows are padded by 8 channels (16 B)
-// against bank conflicts of the 16 pixel lanes.
-constexpr int kLdsTX = 16, kLdsHX = kLdsTX + 2;
+// The same convolution for the three finest levels, tiled and persistent. A
+// tile is TH x 16 output pixels of one image; a block's 4 waves are G
+// output-channel groups x 4/G pixel groups: wave w takes channels
+// [(w % G) COUT/G, +COUT/G) and tile rows (w / G) MT .. + MT - 1 (MT = TH G / 4),
+// 16 pixels each.
+//
+// Each wave loads its weight fragments (9 taps x CIN/32 chunks x COUT/(16 G)
+// channel tiles) into registers once; the block then walks its tiles. A tile's
+// (TH + 2) x 18 input halo, every input channel (zero outside the image; the a
+// source upsampled in place), is copied from global memory straight into LDS
+// (global_load_lds_dwordx4: lane i of a wave writes LDS bytes base + 16 i), into
+// one of two buffers, while the MFMAs run on the other: the next tile's halo is
+// in flight during the current tile's compute. Halo pixels are CIN + 8 channels
+// apart in LDS (the 8 padding channels against bank conflicts of the 16 pixel
+// lanes are filled from a zero buffer, like the padding outside the image).
+// Each step's A fragments are read from LDS one step ahead. With the MFMA's
+// operands exchanged (weights as A, pixels as B) a lane ends with 4 consecutive
+// channels of one pixel, stored as one 8-byte write.
+//
+// The K order per output (tap, then 32-channel chunk) and the operands are those
+// of conv3x3_fwd (with the MFMA's A and B operands exchanged, which yields the
+// transposed tile with every element bit for bit), so the outputs are
+// bit-identical. Tiles are handed to the 8 XCDs in contiguous runs (block b runs
+// on XCD b % 8 and takes tiles of that XCD's run), so neighbouring tiles share
+// their halo rows in one XCD's L2.
+constexpr int kTileX = 16, kHaloX = kTileX + 2;
 
-__host__ __device__ constexpr size_t conv_lds_bytes(int cin, int mt) {
-    return (size_t)(4 * mt + 2) * kLdsHX * (size_t)(cin + 8) * 2;
-}
+template <int CIN, int COUT, int TH>
+struct TileShape {
+    static constexpr int HY = TH + 2;
+    static constexpr int PERP = CIN / 8 + 1;  // 16-byte items per halo pixel, padding included
+    static constexpr int ITEMS = HY * kHaloX * PERP;
+    static constexpr int ROUNDS = (ITEMS + 255) / 256;
+    static constexpr int HALO_BYTES = ROUNDS * 256 * 16;  // one halo buffer
+    static constexpr int LDS_BYTES = 2 * HALO_BYTES;
+};
 
-template <int NT, int MT>
-__global__ void __launch_bounds__(256) conv3x3_fwd_lds(ConvArgs A) {
-    constexpr int kLdsTY = 4 * MT, kLdsHY = kLdsTY + 2;
-    extern __shared__ __align__(16) uint16_t halo[];
-    const int cin = A.ca + A.cb, rs = cin + 8;
-    const int tilesX = (A.W + kLdsTX - 1) / kLdsTX, tilesY = (A.H + kLdsTY - 1) / kLdsTY;
-    int bi = blockIdx.x;
-    const int tx0 = (bi % tilesX) * kLdsTX;
-    bi /= tilesX;
-    const int ty0 = (bi % tilesY) * kLdsTY;
-    const int n = bi / tilesY;
-    const int Hh = A.H >> 1, Wh = A.W >> 1;
-    const int per = cin >> 3;  // 16-byte items per halo pixel
-    const int items = kLdsHY * kLdsHX * per;
-    for (int it = threadIdx.x; it < items; it += 256) {
-        const int hp = it / per, c = (it - hp * per) * 8;
-        const int hy = hp / kLdsHX, hx = hp - hy * kLdsHX;
-        const int ys = ty0 + hy - 1, xs = tx0 + hx - 1;
-        u32x4 v = u32x4{0u, 0u, 0u, 0u};
-        if (ys >= 0 && ys < A.H && xs >= 0 && xs < A.W) {
-            const uint16_t *src;
-            if (c < A.ca) {
-                const size_t pa = A.aUp ? ((size_t)n * Hh + (ys >> 1)) * Wh + (xs >> 1)
-                                        : ((size_t)n * A.H + ys) * A.W + xs;
-                src = A.a + pa * A.ca + c;
-            } else {
-                src = A.b + (((size_t)n * A.H + ys) * A.W + xs) * A.cb + (c - A.ca);
-            }
-            v = *(const u32x4 *)src;
-        }
-        *(u32x4 *)(halo + (size_t)hp * rs + c) = v;
-    }
-    __syncthreads();
+// s_waitcnt with vmcnt = n (0..63) and the other counters left alone.
+#define RC_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
+// s_waitcnt lgkmcnt(0), the other counters left alone.
+#define RC_WAIT_LGKMCNT0() __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4))
+
+template <int CIN, int COUT, int TH, int G>
+__global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
+    static_assert(CIN % 32 == 0 && CIN <= 128 && COUT % (16 * G) == 0, "channels");
+    static_assert((TH * G) % 4 == 0, "rows per wave");
+    using TS = TileShape<CIN, COUT, TH>;
+    constexpr int NTW = COUT / (16 * G);  // channel tiles per wave
+    constexpr int MT = TH * G / 4;        // pixel rows per wave
+    constexpr int RS = CIN + 8;           // LDS stride (bf16) of a halo pixel
+    constexpr int PERP = TS::PERP;
+    constexpr int KC = CIN / 32, STEPS = 9 * KC;
+    static_assert(2 * TS::ROUNDS <= 63, "vmcnt range");
+    // the two halo buffers, as distinct arrays so that the compiler can tell the
+    // copies into one buffer from the reads of the other
+    __shared__ __align__(16) uint16_t halo0[TS::HALO_BYTES / 2];
+    __shared__ __align__(16) uint16_t halo1[TS::HALO_BYTES / 2];
+    const int tilesX = (A.W + kTileX - 1) / kTileX, tilesY = (A.H + TH - 1) / TH;
+    const int tiles = tilesX * tilesY * A.N;
+    const int per = (tiles + 7) >> 3;                    // tiles of one XCD's run
+    const int runBegin = (blockIdx.x & 7) * per;
+    const int runEnd = min(runBegin + per, tiles);
+    const int stride = gridDim.x >> 3;                  // blocks on this XCD
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int kq = 8 * (lane >> 4);
-    f32x4 acc[MT][NT];
+    const int cg = wave % G, pg = wave / G;
+    const int kq = 8 * (lane >> 4);  // this lane's k offset inside a 32-wide chunk
+    const int Hh = A.H >> 1, Wh = A.W >> 1;
+
+    // starts the copies of tile `tile`'s halo into LDS buffer `dst`
+    auto fetch = [&](int tile, uint16_t *dst) {
+        const int tx0 = (tile % tilesX) * kTileX;
+        const int q = tile / tilesX;
+        const int ty0 = (q % tilesY) * TH, n = q / tilesY;
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+        for (int r = 0; r < TS::ROUNDS; ++r) {
+            const int it = r * 256 + (int)threadIdx.x;
+            const int hp = it / PERP, c = (it - hp * PERP) * 8;
+            const int hy = hp / kHaloX, hx = hp - hy * kHaloX;
+            const int ys = ty0 + hy - 1, xs = tx0 + hx - 1;
+            const uint16_t *src = A.zero;
+            if (it < TS::ITEMS && c < CIN && ys >= 0 && ys < A.H && xs >= 0 && xs < A.W) {
+                if (c < A.ca) {
+                    const size_t pa = A.aUp ? ((size_t)n * Hh + (ys >> 1)) * Wh + (xs >> 1)
+                                            : ((size_t)n * A.H + ys) * A.W + xs;
+                    src = A.a + pa * A.ca + c;
+                } else {
+                    src = A.b + (((size_t)n * A.H + ys) * A.W + xs) * A.cb + (c - A.ca);
+                }
+            }
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)src,
+                (__attribute__((address_space(3))) void *)(dst + (size_t)(r * 256 + wave * 64) * 8),
+                16, 0, 0);
+        }
+    };
+
+    // weight fragments of this wave's channels, all steps
+    bf16x8 wf[STEPS][NTW];
+    const uint16_t *wl = A.w + (size_t)(cg * NTW * 16 + (lane & 15)) * 9 * CIN + kq;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3, dx = tap % 3;  // halo offsets of the tap
-        const uint16_t *r0 = halo + ((size_t)(MT * wave + dy) * kLdsHX + (lane & 15) + dx) * rs + kq;
-        const uint16_t *wt = A.w + ((size_t)(lane & 15) * 9 + tap) * cin + kq;
-        for (int c0 = 0; c0 < cin; c0 += 32) {
-            bf16x8 av[MT];
+    for (int st = 0; st < STEPS; ++st)
 #pragma unroll
-            for (int m = 0; m < MT; ++m) av[m] = load8(r0 + (size_t)m * kLdsHX * rs + c0);
+        for (int t = 0; t < NTW; ++t) wf[st][t] = load8(wl + (size_t)t * 16 * 9 * CIN + (st / KC) * CIN + (st % KC) * 32);
+    // this lane's biases: channels co .. co + 3 of each channel tile
+    float bias[NTW][4];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const bf16x8 bv = load8(wt + (size_t)t * 16 * 9 * cin + c0);
+    for (int t = 0; t < NTW; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bias[t][r] = A.bias ? A.bias[(cg * NTW + t) * 16 + 4 * (lane >> 4) + r] : 0.0f;
+    int tile = runBegin + (int)(blockIdx.x >> 3);
+    if (tile < runEnd) fetch(tile, halo0);
+    if (tile + stride < runEnd) {
+        fetch(tile + stride, halo1);
+        RC_WAIT_VMCNT(TS::ROUNDS);  // weights and the first halo
+    } else {
+        RC_WAIT_VMCNT(0);
+    }
+
+    // one tile: compute from `halo` (then refilled with the tile two on)
+    auto step = [&](int tile, uint16_t *halo) {
+        // this wave's copies of `tile` have landed (those of the next tile may
+        // be in flight); the barrier makes every wave's visible
+        if (tile + stride < runEnd) RC_WAIT_VMCNT(TS::ROUNDS);
+        else RC_WAIT_VMCNT(0);
+        __builtin_amdgcn_s_barrier();
+
+        f32x4 acc[MT][NTW];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const uint16_t *hl = halo + ((size_t)(pg * MT) * kHaloX + (lane & 15)) * RS + kq;
+        auto asrc = [&](int st, int m) {
+            const int tap = st / KC;
+            return hl + ((size_t)(m + tap / 3) * kHaloX + tap % 3) * RS + (st % KC) * 32;
+        };
+        bf16x8 av[2][MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) av[0][m] = load8(asrc(0, m));
+#pragma unroll
+        for (int st = 0; st < STEPS; ++st) {
+            if (st + 1 < STEPS) {
+#pragma unroll
+                for (int m = 0; m < MT; ++m) av[(st + 1) & 1][m] = load8(asrc(st + 1, m));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < NTW; ++t)
 #pragma unroll
                 for (int m = 0; m < MT; ++m)
-                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][t], 0, 0, 0);
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[st][t], av[st & 1][m], acc[m][t], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        {
+            const int tx0 = (tile % tilesX) * kTileX;
+            const int q = tile / tilesX;
+            const int ty0 = (q % tilesY) * TH, n = q / tilesY;
+            const int x = tx0 + (lane & 15);
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) {
+                const int co = (cg * NTW + t) * 16 + 4 * (lane >> 4);
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    const int y = ty0 + pg * MT + m;
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        v[r] = acc[m][t][r] + bias[t][r];
+                        if (A.relu) v[r] = fmaxf(v[r], 0.0f);
+                    }
+                    uint2 qq;
+                    qq.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                    qq.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                    if (x < A.W && y < A.H) *(uint2 *)(A.outB + (((size_t)n * A.H + y) * A.W + x) * COUT + co) = qq;
+                }
             }
         }
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + (lane & 15);
-        const float bb = A.bias ? A.bias[co] : 0.0f;
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int y = ty0 + MT * wave + m;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = tx0 + 4 * (lane >> 4) + r;
-                if (x >= A.W || y >= A.H) continue;
-                const size_t p = ((size_t)n * A.H + y) * A.W + x;
-                float v = acc[m][t][r] + bb;
-                if (A.relu) v = fmaxf(v, 0.0f);
-                if (A.outB) A.outB[p * A.cout + co] = f2bf(v);
-                else A.outF[p * A.cout + co] = v;
-            }
-        }
+        RC_WAIT_LGKMCNT0();            // this wave's halo reads are done
+        __builtin_amdgcn_s_barrier();  // ... and every wave's
+        // the buffer just consumed takes the halo two tiles on
+        if (tile + 2 * stride < runEnd) fetch(tile + 2 * stride, halo);
+    };
+    while (tile < runEnd) {
+        step(tile, halo0);
+        tile += stride;
+        if (tile >= runEnd) break;
+        step(tile, halo1);
+        tile += stride;
     }
 }
 
@@ -783,6 +884,7 @@ struct Net {
     int wbTotal = 0, wtTotal = 0;
     float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     uint16_t *wb = nullptr, *wt = nullptr;
+    uint16_t *zero = nullptr;  // 256 zero bf16: the source of the tiled convolution's padding loads
     int step = 0;
     bool ready = false;
     // dataset
@@ -926,26 +1028,37 @@ void launch_conv_nt(hipStream_t s, const ConvArgs &A) {
 // its output channels over the waves of a block.
 constexpr size_t kFewTiles = 4096;
 
-// conv3x3_fwd_lds for 32 or 64 output channels and at most 128 input channels:
-// 16 x 16 tiles (MT 4) up to 64 input channels, 8 x 16 tiles (MT 2) above
-// (LDS per block at most 49 KB); false when it does not apply.
-bool launch_conv_lds(hipStream_t s, const ConvArgs &A) {
+// conv3x3_tile for the (input, output) channel counts of the three finest
+// levels: e1 32/32, e2 32/64, e3 64/64, d3 128/64, d2 128/32, d1 64/32; false
+// for any other shape.
+template <int CIN, int COUT, int TH, int G>
+void launch_tile(hipStream_t s, const ConvArgs &A) {
+    using TS = TileShape<CIN, COUT, TH>;
+    const int tiles = A.N * ((A.H + TH - 1) / TH) * ((A.W + kTileX - 1) / kTileX);
+    const int per = (tiles + 7) / 8;
+    // blocks per XCD: as many as fit on its 32 CUs at once (LDS-bound), at most
+    // one per tile of its run
+    const int perCu = std::max(1, (int)(160 * 1024 / TS::LDS_BYTES));
+    const int bpx = std::min(per, 32 * perCu);
+    hipLaunchKernelGGL((conv3x3_tile<CIN, COUT, TH, G>), dim3(8 * bpx), dim3(256), 0, s, A);
+}
+
+bool launch_conv_tiled(hipStream_t s, const ConvArgs &A) {
     const int cin = A.ca + A.cb;
-    if ((A.cout != 32 && A.cout != 64) || cin > 128 || (cin & 31) || (A.ca & 7) || (A.cb & 7)) return false;
-    const int mt = cin <= 64 ? 4 : 2;
-    const unsigned tiles = (unsigned)A.N * (unsigned)((A.H + 4 * mt - 1) / (4 * mt)) *
-                           (unsigned)((A.W + kLdsTX - 1) / kLdsTX);
-    const size_t lds = conv_lds_bytes(cin, mt);
-    if (A.cout == 32 && mt == 4) hipLaunchKernelGGL((conv3x3_fwd_lds<2, 4>), dim3(tiles), dim3(256), lds, s, A);
-    else if (A.cout == 32) hipLaunchKernelGGL((conv3x3_fwd_lds<2, 2>), dim3(tiles), dim3(256), lds, s, A);
-    else if (mt == 4) hipLaunchKernelGGL((conv3x3_fwd_lds<4, 4>), dim3(tiles), dim3(256), lds, s, A);
-    else hipLaunchKernelGGL((conv3x3_fwd_lds<4, 2>), dim3(tiles), dim3(256), lds, s, A);
+    if ((A.ca & 7) || (A.cb & 7) || !A.outB || !A.zero) return false;
+    if (cin == 32 && A.cout == 32) launch_tile<32, 32, 8, 2>(s, A);
+    else if (cin == 32 && A.cout == 64) launch_tile<32, 64, 8, 4>(s, A);
+    else if (cin == 64 && A.cout == 64) launch_tile<64, 64, 8, 4>(s, A);
+    else if (cin == 64 && A.cout == 32) launch_tile<64, 32, 8, 2>(s, A);
+    else if (cin == 128 && A.cout == 32) launch_tile<128, 32, 4, 2>(s, A);
+    else if (cin == 128 && A.cout == 64) launch_tile<128, 64, 4, 4>(s, A);
+    else return false;
     return true;
 }
 
-hipError_t launch_conv(hipStream_t s, const ConvArgs &A) {
+hipError_t launch_conv(hipStream_t s, const ConvArgs &A, bool allowGroups = true) {
     if (A.cout % 16) return hipErrorInvalidValue;
-    const bool few = ((size_t)A.N * A.H * A.W + 31) / 32 < kFewTiles;
+    const bool few = allowGroups && ((size_t)A.N * A.H * A.W + 31) / 32 < kFewTiles;
     switch (A.cout / 16) {
         case 2: few ? launch_conv_nt<1, 2>(s, A) : launch_conv_nt<2, 1>(s, A); break;
         case 4: few ? launch_conv_nt<1, 4>(s, A) : launch_conv_nt<4, 1>(s, A); break;
@@ -989,9 +1102,10 @@ hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W) {
         A.cout = sp.cout;
         A.relu = 1;
         A.outB = n.act[L];
+        A.zero = n.zero;
         // the three finest levels stage their input tiles in LDS
-        if (sp.level <= 2 && launch_conv_lds(s, A)) continue;
-        hipError_t e = launch_conv(s, A);
+        if (ctx->netKernels == RC_NET_KERNELS_TILED && sp.level <= 2 && launch_conv_tiled(s, A)) continue;
+        hipError_t e = launch_conv(s, A, ctx->netKernels != RC_NET_KERNELS_SINGLE_GROUP);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();
@@ -1162,6 +1276,7 @@ hipError_t alloc_params(Net &n) {
     if (e == hipSuccess && !n.grad) e = nalloc(n.grad, n.nParams);
     if (e == hipSuccess && !n.wb) e = nalloc(n.wb, n.wbTotal);
     if (e == hipSuccess && !n.wt) e = nalloc(n.wt, n.wtTotal);
+    if (e == hipSuccess && !n.zero) e = nalloc(n.zero, 256);
     return e;
 }
 
@@ -1171,6 +1286,7 @@ int set_params_host(rc_ctx *ctx, Net &n, const float *theta) {
     if (e == hipSuccess) e = hipMemsetAsync(n.m, 0, (size_t)n.nParams * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(n.v, 0, (size_t)n.nParams * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(n.wt, 0, (size_t)n.wtTotal * 2, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(n.zero, 0, 256 * 2, ctx->stream);
     if (e == hipSuccess) e = prep(ctx, n);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "net parameter upload");
@@ -1184,7 +1300,7 @@ int set_params_host(rc_ctx *ctx, Net &n, const float *theta) {
 void free_net(Net *n) {
     if (!n) return;
     free_ws(*n);
-    nfree(n->theta); nfree(n->m); nfree(n->v); nfree(n->grad); nfree(n->wb); nfree(n->wt);
+    nfree(n->theta); nfree(n->m); nfree(n->v); nfree(n->grad); nfree(n->wb); nfree(n->wt); nfree(n->zero);
     nfree(n->raw); nfree(n->tgt); nfree(n->lossDev);
     delete n;
 }
@@ -1434,6 +1550,15 @@ int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t 
         out[3 * i + 1] = tmp[i].y;
         out[3 * i + 2] = tmp[i].z;
     }
+    return RC_OK;
+}
+
+int rc_set_net_kernels(rc_ctx *ctx, int32_t mode) {
+    if (!ctx) return RC_ERR_INVALID_ARG;
+    if (mode != RC_NET_KERNELS_TILED && mode != RC_NET_KERNELS_DIRECT && mode != RC_NET_KERNELS_SINGLE_GROUP)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "bad network kernel mode");
+    if (int rc_ = capture_block(ctx, "rc_set_net_kernels")) return rc_;
+    ctx->netKernels = mode;
     return RC_OK;
 }
 

@@ -96,6 +96,11 @@ class NetDenoiser:
         the frame and its mirror images."""
         self.ctx.check(self._lib.rc_set_net_flips(self.ctx.h, int(passes)))
 
+    def set_kernels(self, mode: int = N.RC_NET_KERNELS_TILED):
+        """rc_set_net_kernels: which convolution kernels the forward pass uses
+        (tiled, direct or direct with one channel group); outputs are identical."""
+        self.ctx.check(self._lib.rc_set_net_kernels(self.ctx.h, int(mode)))
+
     def enable(self, on: bool = True):
         """rc_post's denoise step runs the network (on) or a-trous (off)."""
         self.ctx.check(self._lib.rc_set_denoiser(self.ctx.h, N.RC_DENOISE_NET if on else N.RC_DENOISE_ATROUS))

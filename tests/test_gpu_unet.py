@@ -70,6 +70,31 @@ def test_gradient_matches_oracle(gpu_ctx):
         assert rel_b <= 3e-2, (name, "bias", rel_b)
 
 
+@pytest.mark.parametrize("shape", [(4, 480, 864), (8, 128, 128), (1, 48, 80), (3, 16, 16)])
+def test_conv_kernels_are_bit_identical(gpu_ctx, shape):
+    # The tiled kernels (input tiles in LDS, weight fragments prefetched, tiles
+    # grouped per XCD), the direct kernels and the direct kernels with one
+    # channel group per wave compute every output from the same operands in the
+    # same K order: the network's outputs and gradients agree bit for bit.
+    n, h, w = shape
+    net = NetDenoiser(gpu_ctx).load()
+    raw, tgt = random_raw(20 + n, n, h, w)
+    outs, grads = [], []
+    try:
+        for mode in (N.RC_NET_KERNELS_TILED, N.RC_NET_KERNELS_DIRECT, N.RC_NET_KERNELS_SINGLE_GROUP):
+            net.set_kernels(mode)
+            outs.append(net.forward(raw))
+            if n * h * w <= 8 * 128 * 128:
+                grads.append(net.grad(raw, tgt)[1])
+    finally:
+        net.set_kernels(N.RC_NET_KERNELS_TILED)
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+    for g in grads[1:]:
+        assert np.array_equal(g.view(np.uint32), grads[0].view(np.uint32))
+    assert np.isfinite(outs[0]).all() and outs[0].max() > 0
+
+
 def test_training_reduces_loss(gpu_ctx):
     raw, tgt = make_dataset(gpu_ctx, range(6), size=64, spp_in=(8,), spp_ref=256)
     net = NetDenoiser(gpu_ctx).init(3)
