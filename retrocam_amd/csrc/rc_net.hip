@@ -116,6 +116,7 @@ struct ConvArgs {
     uint16_t *outB;     // bf16 output, or
     float *outF;        // f32 output
     const uint16_t *zero;  // >= 8 zero bf16 (conv3x3_tile's padding source)
+    uint16_t *pool;        // conv3x3_tile: also the 2 x 2 mean of the output (pool2), or null
 };
 
 template <int NT, int MT, int G>
@@ -360,19 +361,43 @@ __global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
 #pragma unroll
             for (int t = 0; t < NTW; ++t) {
                 const int co = (cg * NTW + t) * 16 + 4 * (lane >> 4);
+                uint16_t b[MT][4];  // the stored bfloat16 outputs
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const int y = ty0 + pg * MT + m;
-                    float v[4];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        v[r] = acc[m][t][r] + bias[t][r];
-                        if (A.relu) v[r] = fmaxf(v[r], 0.0f);
+                        float v = acc[m][t][r] + bias[t][r];
+                        if (A.relu) v = fmaxf(v, 0.0f);
+                        b[m][r] = f2bf(v);
                     }
                     uint2 qq;
-                    qq.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                    qq.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                    qq.x = (uint32_t)b[m][0] | ((uint32_t)b[m][1] << 16);
+                    qq.y = (uint32_t)b[m][2] | ((uint32_t)b[m][3] << 16);
                     if (x < A.W && y < A.H) *(uint2 *)(A.outB + (((size_t)n * A.H + y) * A.W + x) * COUT + co) = qq;
+                }
+                // 2 x 2 mean of the stored values for the next level, as pool2
+                // computes it: ((y, x) + (y, x + 1)) + ((y + 1, x) + (y + 1, x + 1)),
+                // the x + 1 column taken from the neighbouring lane
+                if (A.pool && (MT % 2) == 0) {
+                    const int Ho = A.H >> 1, Wo = A.W >> 1;
+#pragma unroll
+                    for (int m = 0; m + 1 < MT; m += 2) {
+                        const int yo = (ty0 + pg * MT + m) >> 1, xo = x >> 1;
+                        float s[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const float a0 = bf2f(b[m][r]), b0 = bf2f(b[m + 1][r]);
+                            const float a1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(a0), 0xB1, 0xF, 0xF, false));
+                            const float b1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(b0), 0xB1, 0xF, 0xF, false));
+                            s[r] = ((a0 + a1) + (b0 + b1)) * 0.25f;
+                        }
+                        uint2 qq;
+                        qq.x = (uint32_t)f2bf(s[0]) | ((uint32_t)f2bf(s[1]) << 16);
+                        qq.y = (uint32_t)f2bf(s[2]) | ((uint32_t)f2bf(s[3]) << 16);
+                        if (!(lane & 1) && xo < Wo && yo < Ho)
+                            *(uint2 *)(A.pool + (((size_t)n * Ho + yo) * Wo + xo) * COUT + co) = qq;
+                    }
                 }
             }
         }
@@ -1032,8 +1057,10 @@ constexpr size_t kFewTiles = 4096;
 // levels: e1 32/32, e2 32/64, e3 64/64, d3 128/64, d2 128/32, d1 64/32; false
 // for any other shape.
 template <int CIN, int COUT, int TH, int G>
-void launch_tile(hipStream_t s, const ConvArgs &A) {
+void launch_tile(hipStream_t s, ConvArgs A, bool &pooled) {
     using TS = TileShape<CIN, COUT, TH>;
+    if ((TH * G / 4) % 2) A.pool = nullptr;  // the pooled output needs row pairs per wave
+    pooled = A.pool != nullptr;
     const int tiles = A.N * ((A.H + TH - 1) / TH) * ((A.W + kTileX - 1) / kTileX);
     const int per = (tiles + 7) / 8;
     // blocks per XCD: as many as fit on its 32 CUs at once (LDS-bound), at most
@@ -1043,15 +1070,19 @@ void launch_tile(hipStream_t s, const ConvArgs &A) {
     hipLaunchKernelGGL((conv3x3_tile<CIN, COUT, TH, G>), dim3(8 * bpx), dim3(256), 0, s, A);
 }
 
-bool launch_conv_tiled(hipStream_t s, const ConvArgs &A) {
+// pooled: whether the launch also wrote A.pool (the 2 x 2 mean of the output;
+// H and W even).
+bool launch_conv_tiled(hipStream_t s, const ConvArgs &A, bool &pooled) {
     const int cin = A.ca + A.cb;
+    pooled = false;
     if ((A.ca & 7) || (A.cb & 7) || !A.outB || !A.zero) return false;
-    if (cin == 32 && A.cout == 32) launch_tile<32, 32, 8, 2>(s, A);
-    else if (cin == 32 && A.cout == 64) launch_tile<32, 64, 8, 4>(s, A);
-    else if (cin == 64 && A.cout == 64) launch_tile<64, 64, 8, 4>(s, A);
-    else if (cin == 64 && A.cout == 32) launch_tile<64, 32, 8, 2>(s, A);
-    else if (cin == 128 && A.cout == 32) launch_tile<128, 32, 4, 2>(s, A);
-    else if (cin == 128 && A.cout == 64) launch_tile<128, 64, 4, 4>(s, A);
+    if (A.pool && ((A.H | A.W) & 1)) return false;
+    if (cin == 32 && A.cout == 32) launch_tile<32, 32, 8, 2>(s, A, pooled);
+    else if (cin == 32 && A.cout == 64) launch_tile<32, 64, 8, 4>(s, A, pooled);
+    else if (cin == 64 && A.cout == 64) launch_tile<64, 64, 8, 4>(s, A, pooled);
+    else if (cin == 64 && A.cout == 32) launch_tile<64, 32, 8, 2>(s, A, pooled);
+    else if (cin == 128 && A.cout == 32) launch_tile<128, 32, 4, 2>(s, A, pooled);
+    else if (cin == 128 && A.cout == 64) launch_tile<128, 64, 4, 4>(s, A, pooled);
     else return false;
     return true;
 }
@@ -1081,9 +1112,10 @@ void layer_inputs(Net &n, int L, const uint16_t *&a, const uint16_t *&b) {
 
 hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W) {
     hipStream_t s = ctx->stream;
+    bool pooledBy[kNumLayers] = {};  // the layer's convolution wrote its pooled output
     for (int L = 0; L < 9; ++L) {
         const LayerSpec &sp = kLayers[L];
-        if (L >= 1 && L <= 4) {  // pool the previous encoder output
+        if (L >= 1 && L <= 4 && !pooledBy[L - 1]) {  // pool the previous encoder output
             const int lv = kLayers[L - 1].level;
             const size_t cnt = level_pix(N, H, W, lv + 1) * kLayers[L - 1].cout;
             hipLaunchKernelGGL(pool2, dim3(blocks_for(cnt)), dim3(256), 0, s, n.act[L - 1], N, H >> lv, W >> lv,
@@ -1103,8 +1135,10 @@ hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W) {
         A.relu = 1;
         A.outB = n.act[L];
         A.zero = n.zero;
+        // the encoders of the three finest levels also write their pooled output
+        A.pool = L <= 2 ? n.pooled[L] : nullptr;
         // the three finest levels stage their input tiles in LDS
-        if (ctx->netKernels == RC_NET_KERNELS_TILED && sp.level <= 2 && launch_conv_tiled(s, A)) continue;
+        if (ctx->netKernels == RC_NET_KERNELS_TILED && sp.level <= 2 && launch_conv_tiled(s, A, pooledBy[L])) continue;
         hipError_t e = launch_conv(s, A, ctx->netKernels != RC_NET_KERNELS_SINGLE_GROUP);
         if (e != hipSuccess) return e;
     }
