@@ -80,6 +80,18 @@ typedef struct rc_light {            /* 80 B, SceneUploader.java:173-205       *
     float normal[3]; float cdf;      /* cdf: cumulative power fraction, last = 1 */
 } rc_light;
 
+/* The record sizes the Java host writes (std430 strides). */
+#ifdef __cplusplus
+#define RC_SIZE_CHECK(t, n) static_assert(sizeof(t) == (n), #t " must be " #n " bytes")
+#else
+#define RC_SIZE_CHECK(t, n) _Static_assert(sizeof(t) == (n), #t " must be " #n " bytes")
+#endif
+RC_SIZE_CHECK(rc_gpu_triangle, 128);
+RC_SIZE_CHECK(rc_bvh_node, 48);
+RC_SIZE_CHECK(rc_material, 64);
+RC_SIZE_CHECK(rc_light, 80);
+#undef RC_SIZE_CHECK
+
 /* Camera + lens uniforms of one pass (ThinLensCamera.uploadTo,
  * src/main/java/com/retrocam/camera/ThinLensCamera.java:27-58, plus the render
  * resolution that Renderer.java:100-101 sets). */
