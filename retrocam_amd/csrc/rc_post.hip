@@ -659,11 +659,18 @@ static void fill_weights(PostWeights &W, const rc_post_params &p) {
 
 enum { P_P00 = 0, P_ATROUS = 100, P_DISPLAY = 200 };
 
+// One pass over the image in 16 x 16 tiles. The tiles (row-major) are handed to
+// the 8 XCDs in contiguous runs (dispatch slot b goes to XCD b % 8), so each
+// XCD works through a band of rows and the neighbourhoods its taps read (the
+// a-trous 5 x 5 at step up to 16, smears, blurs) stay in that XCD's L2.
 template <int PASS>
 __global__ void __launch_bounds__(256) post_kernel(PostArgs A) {
     const SweepView V{A.p, A.dt ? *A.dt : A.t, A.g, A.step, A.wts};
-    int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int tilesX = (A.src.w + 15) >> 4, tiles = tilesX * ((A.src.h + 15) >> 4);
+    const int tile = (int)(blockIdx.x & 7) * ((tiles + 7) >> 3) + (int)(blockIdx.x >> 3);
+    if (tile >= tiles) return;
+    int x = (tile % tilesX) * 16 + (threadIdx.x & 15);
+    int y = (tile / tilesX) * 16 + (threadIdx.x >> 4);
     if (x >= A.src.w || y >= A.src.h) return;
     size_t i = (size_t)y * (size_t)A.src.w + (size_t)x;
     if (PASS == P_DISPLAY) {
@@ -742,8 +749,8 @@ hipError_t launch_post_pass(hipStream_t stream, int pass, const float4 *src, flo
     A.step = step;
     A.rgba8 = rgba8;
     fill_weights(A.wts, p);
-    dim3 grid((w + 15) / 16, (h + 15) / 16);
-    hipLaunchKernelGGL(k, grid, dim3(256), 0, stream, A);
+    const unsigned tiles = (unsigned)((w + 15) / 16) * (unsigned)((h + 15) / 16);
+    hipLaunchKernelGGL(k, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, stream, A);
     return hipGetLastError();
 }
 
