@@ -117,7 +117,20 @@ struct ConvArgs {
     float *outF;        // f32 output
     const uint16_t *zero;  // >= 8 zero bf16 (conv3x3_tile's padding source)
     uint16_t *pool;        // conv3x3_tile: also the 2 x 2 mean of the output (pool2), or null
+    // mirW > 0: a holds ONE image (N = 1 layout) of a w x h frame padded to W x H
+    // by edge replication; image n of the batch is that frame mirrored by n (bit
+    // 0 left-right, bit 1 top-bottom), read in place
+    int mirW, mirH;
 };
+
+// Pixel of the single stored image that image n's pixel (y, x) reads when the
+// batch is the mirror ensemble of one padded w x h frame (ConvArgs::mirW).
+__device__ __forceinline__ size_t mirror_src(int n, int y, int x, int W, int w, int h) {
+    int xb = min(x, w - 1), yb = min(y, h - 1);
+    if (n & 1) xb = w - 1 - xb;
+    if (n & 2) yb = h - 1 - yb;
+    return (size_t)yb * W + xb;
+}
 
 template <int NT, int MT, int G>
 __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
@@ -158,8 +171,9 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
             inb[m] = pv[m] && ys >= 0 && ys < A.H && xs >= 0 && xs < A.W;
             ys = inb[m] ? ys : 0;
             xs = inb[m] ? xs : 0;
-            size_t pa = A.aUp ? ((size_t)pn[m] * Hh + (ys >> 1)) * Wh + (xs >> 1)
-                              : ((size_t)pn[m] * A.H + ys) * A.W + xs;
+            size_t pa = A.aUp    ? ((size_t)pn[m] * Hh + (ys >> 1)) * Wh + (xs >> 1)
+                        : A.mirW ? mirror_src(pn[m], ys, xs, A.W, A.mirW, A.mirH)
+                                 : ((size_t)pn[m] * A.H + ys) * A.W + xs;
             srcA[m] = A.a + pa * A.ca + kq;
             srcB[m] = A.b ? A.b + (((size_t)pn[m] * A.H + ys) * A.W + xs) * A.cb + kq : nullptr;
         }
@@ -268,7 +282,7 @@ __global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
     const int Hh = A.H >> 1, Wh = A.W >> 1;
 
     // starts the copies of tile `tile`'s halo into LDS buffer `dst`
-    auto fetch = [&](int tile, uint16_t *dst) {
+    auto fetch = [&](int tile, uint16_t *dst) __attribute__((always_inline)) {
         const int tx0 = (tile % tilesX) * kTileX;
         const int q = tile / tilesX;
         const int ty0 = (q % tilesY) * TH, n = q / tilesY;
@@ -281,8 +295,9 @@ __global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
             const uint16_t *src = A.zero;
             if (it < TS::ITEMS && c < CIN && ys >= 0 && ys < A.H && xs >= 0 && xs < A.W) {
                 if (c < A.ca) {
-                    const size_t pa = A.aUp ? ((size_t)n * Hh + (ys >> 1)) * Wh + (xs >> 1)
-                                            : ((size_t)n * A.H + ys) * A.W + xs;
+                    const size_t pa = A.aUp    ? ((size_t)n * Hh + (ys >> 1)) * Wh + (xs >> 1)
+                                      : A.mirW ? mirror_src(n, ys, xs, A.W, A.mirW, A.mirH)
+                                               : ((size_t)n * A.H + ys) * A.W + xs;
                     src = A.a + pa * A.ca + c;
                 } else {
                     src = A.b + (((size_t)n * A.H + ys) * A.W + xs) * A.cb + (c - A.ca);
@@ -318,7 +333,7 @@ __global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
     }
 
     // one tile: compute from `halo` (then refilled with the tile two on)
-    auto step = [&](int tile, uint16_t *halo) {
+    auto step = [&](int tile, uint16_t *halo) __attribute__((always_inline)) {
         // this wave's copies of `tile` have landed (those of the next tile may
         // be in flight); the barrier makes every wave's visible
         if (tile + stride < runEnd) RC_WAIT_VMCNT(TS::ROUNDS);
@@ -667,7 +682,8 @@ __device__ __forceinline__ float head_channel(const uint16_t *wb, const float *b
 }
 
 // Head of a flip ensemble: the batch holds `passes` images of (Hp, Wp), image k
-// the frame mirrored by k (bit 0 left-right, bit 1 top-bottom). Pixel (x, y) of
+// the frame mirrored by k (bit 0 left-right, bit 1 top-bottom); the features are
+// the frame's, stored once. Pixel (x, y) of
 // the w x h output is ((v0 + v1) + v2 + v3) * (1 / passes) of the clamped head
 // outputs vk at the mirrored positions; one pass writes v0 itself.
 struct EnsembleArgs {
@@ -680,6 +696,9 @@ __global__ void __launch_bounds__(256) head_ensemble_kernel(EnsembleArgs A) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= A.w * A.h) return;
     const int x = i % A.w, y = i / A.w;
+    // the features are stored once (image 0); image k's pixel (xk, yk) is the
+    // frame's (x, y)
+    const size_t pf = (size_t)y * A.Wp + x;
     float acc[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int k = 0; k < A.passes; ++k) {
@@ -691,7 +710,7 @@ __global__ void __launch_bounds__(256) head_ensemble_kernel(EnsembleArgs A) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             float yy, m;
-            const float v = fmaxf(head_channel(A.wb, A.bias, A.feat, p, j, d1v, yy, m), 0.0f);
+            const float v = fmaxf(head_channel(A.wb, A.bias, A.feat, pf, j, d1v, yy, m), 0.0f);
             acc[j] = k == 0 ? v : acc[j] + v;
         }
     }
@@ -842,17 +861,13 @@ __global__ void raw_features(const float *raw, size_t npix, uint16_t *feat) {
 }
 // Features of a rendered frame, padded to (Hp, Wp) by edge replication:
 // radiance from the post image (p00 output), albedo/normal/depth from the
-// RGBA16F G-buffers, mean luma^2 = variance sum / spp. Image k of the
-// `passes` images written is the frame mirrored by k (bit 0 left-right, bit 1
-// top-bottom) before the padding.
+// RGBA16F G-buffers, mean luma^2 = variance sum / spp. One image: the mirrored
+// images of a flip ensemble read it in place (ConvArgs::mirW).
 __global__ void frame_features(const float4 *hdr, const ushort4 *gbuf, const ushort4 *galb, const float *variance,
-                               int total, int w, int h, int Wp, int Hp, int passes, uint16_t *feat) {
+                               int total, int w, int h, int Wp, int Hp, uint16_t *feat) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= passes * Wp * Hp) return;
-    const int k = i / (Wp * Hp), j = i - k * (Wp * Hp);
-    int x = min(j % Wp, w - 1), y = min(j / Wp, h - 1);
-    if (k & 1) x = w - 1 - x;
-    if (k & 2) y = h - 1 - y;
+    if (i >= Wp * Hp) return;
+    const int x = min(i % Wp, w - 1), y = min(i / Wp, h - 1);
     const size_t s = (size_t)y * w + x;
     float4 c = hdr[s];
     ushort4 g = gbuf[s], a = galb[s];
@@ -1110,7 +1125,9 @@ void layer_inputs(Net &n, int L, const uint16_t *&a, const uint16_t *&b) {
     b = s.srcB >= 0 ? n.act[s.srcB] : nullptr;
 }
 
-hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W) {
+// mirW, mirH > 0: the features hold one padded mirW x mirH frame and image n of
+// the batch is that frame mirrored by n (a flip ensemble).
+hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W, int mirW = 0, int mirH = 0) {
     hipStream_t s = ctx->stream;
     bool pooledBy[kNumLayers] = {};  // the layer's convolution wrote its pooled output
     for (int L = 0; L < 9; ++L) {
@@ -1135,6 +1152,10 @@ hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W) {
         A.relu = 1;
         A.outB = n.act[L];
         A.zero = n.zero;
+        if (sp.srcA < 0) {  // the features
+            A.mirW = mirW;
+            A.mirH = mirH;
+        }
         // the encoders of the three finest levels also write their pooled output
         A.pool = L <= 2 ? n.pooled[L] : nullptr;
         // the three finest levels stage their input tiles in LDS
@@ -1355,9 +1376,9 @@ int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const u
         return set_error(ctx, RC_ERR_LIMIT, "net denoise: passes x padded frame above 64 Mpixels (lower rc_set_net_flips)");
     hipError_t e = ensure_ws(ctx, n, passes, Hp, Wp, false);
     if (e != hipSuccess) return hip_fail(ctx, e, "net workspace");
-    hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)passes * Wp * Hp)), dim3(256), 0, ctx->stream, src,
-                       gbuf, galbedo, variance, total, w, h, Wp, Hp, passes, n.feat);
-    if ((e = forward(ctx, n, passes, Hp, Wp)) != hipSuccess) return hip_fail(ctx, e, "net forward");
+    hipLaunchKernelGGL(frame_features, dim3(blocks_for((size_t)Wp * Hp)), dim3(256), 0, ctx->stream, src, gbuf,
+                       galbedo, variance, total, w, h, Wp, Hp, n.feat);
+    if ((e = forward(ctx, n, passes, Hp, Wp, w, h)) != hipSuccess) return hip_fail(ctx, e, "net forward");
     EnsembleArgs hd{};
     hd.d1 = n.act[8];
     hd.feat = n.feat;
