@@ -35,10 +35,10 @@
 //                       read in place — no concat or upsample buffers. Also the
 //                       input gradient: dX = W' * dZ with W' the tap-flipped,
 //                       transposed weights.
-//   conv3x3_tile<CIN,COUT,TH,G>  the forward convolution of the three finest
-//                       levels: weight fragments in registers, each block's
-//                       input tile staged in LDS (same K order and operands:
-//                       bit-identical outputs).
+//   conv3x3_tile<CIN,COUT,TH,G,WV>  the forward convolution of the layers of at
+//                       most 128 input channels: weight fragments in registers,
+//                       each block's input tile staged in LDS (same K order and
+//                       operands: bit-identical outputs).
 //   conv3x3_dw          dW[co][tap][ci] = sum_p dZ[p][co] X[p+tap][ci]: a GEMM
 //                       with K = pixels, operands transposed through LDS,
 //                       per-chunk partials reduced in a fixed order
@@ -213,10 +213,10 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
     }
 }
 
-// The same convolution for the three finest levels, tiled and persistent. A
-// tile is TH x 16 output pixels of one image; a block's 4 waves are G
-// output-channel groups x 4/G pixel groups: wave w takes channels
-// [(w % G) COUT/G, +COUT/G) and tile rows (w / G) MT .. + MT - 1 (MT = TH G / 4),
+// The same convolution for every layer of at most 128 input channels, tiled
+// and persistent. A tile is TH x 16 output pixels of one image; a block's WV
+// waves are G output-channel groups x WV/G pixel groups: wave w takes channels
+// [(w % G) COUT/G, +COUT/G) and tile rows (w / G) MT .. + MT - 1 (MT = TH G / WV),
 // 16 pixels each.
 //
 // Each wave loads its weight fragments (9 taps x CIN/32 chunks x COUT/(16 G)
@@ -240,13 +240,13 @@ __global__ void __launch_bounds__(256) conv3x3_fwd(ConvArgs A) {
 // their halo rows in one XCD's L2.
 constexpr int kTileX = 16, kHaloX = kTileX + 2;
 
-template <int CIN, int COUT, int TH>
+template <int CIN, int COUT, int TH, int THREADS>
 struct TileShape {
     static constexpr int HY = TH + 2;
     static constexpr int PERP = CIN / 8 + 1;  // 16-byte items per halo pixel, padding included
     static constexpr int ITEMS = HY * kHaloX * PERP;
-    static constexpr int ROUNDS = (ITEMS + 255) / 256;
-    static constexpr int HALO_BYTES = ROUNDS * 256 * 16;  // one halo buffer
+    static constexpr int ROUNDS = (ITEMS + THREADS - 1) / THREADS;
+    static constexpr int HALO_BYTES = ROUNDS * THREADS * 16;  // one halo buffer
     static constexpr int LDS_BYTES = 2 * HALO_BYTES;
 };
 
@@ -255,13 +255,14 @@ struct TileShape {
 // s_waitcnt lgkmcnt(0), the other counters left alone.
 #define RC_WAIT_LGKMCNT0() __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4))
 
-template <int CIN, int COUT, int TH, int G>
-__global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
+template <int CIN, int COUT, int TH, int G, int WV>
+__global__ void __launch_bounds__(64 * WV) conv3x3_tile(ConvArgs A) {
     static_assert(CIN % 32 == 0 && CIN <= 128 && COUT % (16 * G) == 0, "channels");
-    static_assert((TH * G) % 4 == 0, "rows per wave");
-    using TS = TileShape<CIN, COUT, TH>;
+    static_assert(WV % G == 0 && (TH * G) % WV == 0, "waves and rows per wave");
+    constexpr int THREADS = 64 * WV;
+    using TS = TileShape<CIN, COUT, TH, THREADS>;
     constexpr int NTW = COUT / (16 * G);  // channel tiles per wave
-    constexpr int MT = TH * G / 4;        // pixel rows per wave
+    constexpr int MT = TH * G / WV;       // pixel rows per wave
     constexpr int RS = CIN + 8;           // LDS stride (bf16) of a halo pixel
     constexpr int PERP = TS::PERP;
     constexpr int KC = CIN / 32, STEPS = 9 * KC;
@@ -288,7 +289,7 @@ __global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
         const int ty0 = (q % tilesY) * TH, n = q / tilesY;
 #pragma unroll
         for (int r = 0; r < TS::ROUNDS; ++r) {
-            const int it = r * 256 + (int)threadIdx.x;
+            const int it = r * THREADS + (int)threadIdx.x;
             const int hp = it / PERP, c = (it - hp * PERP) * 8;
             const int hy = hp / kHaloX, hx = hp - hy * kHaloX;
             const int ys = ty0 + hy - 1, xs = tx0 + hx - 1;
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(256) conv3x3_tile(ConvArgs A) {
             }
             __builtin_amdgcn_global_load_lds(
                 (const __attribute__((address_space(1))) void *)src,
-                (__attribute__((address_space(3))) void *)(dst + (size_t)(r * 256 + wave * 64) * 8),
+                (__attribute__((address_space(3))) void *)(dst + (size_t)(r * THREADS + wave * 64) * 8),
                 16, 0, 0);
         }
     };
@@ -1071,10 +1072,10 @@ constexpr size_t kFewTiles = 4096;
 // conv3x3_tile for the (input, output) channel counts of the three finest
 // levels: e1 32/32, e2 32/64, e3 64/64, d3 128/64, d2 128/32, d1 64/32; false
 // for any other shape.
-template <int CIN, int COUT, int TH, int G>
+template <int CIN, int COUT, int TH, int G, int WV = 4>
 void launch_tile(hipStream_t s, ConvArgs A, bool &pooled) {
-    using TS = TileShape<CIN, COUT, TH>;
-    if ((TH * G / 4) % 2) A.pool = nullptr;  // the pooled output needs row pairs per wave
+    using TS = TileShape<CIN, COUT, TH, 64 * WV>;
+    if ((TH * G / WV) % 2) A.pool = nullptr;  // the pooled output needs row pairs per wave
     pooled = A.pool != nullptr;
     const int tiles = A.N * ((A.H + TH - 1) / TH) * ((A.W + kTileX - 1) / kTileX);
     const int per = (tiles + 7) / 8;
@@ -1082,7 +1083,7 @@ void launch_tile(hipStream_t s, ConvArgs A, bool &pooled) {
     // one per tile of its run
     const int perCu = std::max(1, (int)(160 * 1024 / TS::LDS_BYTES));
     const int bpx = std::min(per, 32 * perCu);
-    hipLaunchKernelGGL((conv3x3_tile<CIN, COUT, TH, G>), dim3(8 * bpx), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((conv3x3_tile<CIN, COUT, TH, G, WV>), dim3(8 * bpx), dim3(64 * WV), 0, s, A);
 }
 
 // pooled: whether the launch also wrote A.pool (the 2 x 2 mean of the output;
@@ -1098,6 +1099,8 @@ bool launch_conv_tiled(hipStream_t s, const ConvArgs &A, bool &pooled) {
     else if (cin == 64 && A.cout == 32) launch_tile<64, 32, 8, 2>(s, A, pooled);
     else if (cin == 128 && A.cout == 32) launch_tile<128, 32, 4, 2>(s, A, pooled);
     else if (cin == 128 && A.cout == 64) launch_tile<128, 64, 4, 4>(s, A, pooled);
+    else if (cin == 64 && A.cout == 96) launch_tile<64, 96, 4, 6, 6>(s, A, pooled);    // e4: 6 channel groups
+    else if (cin == 96 && A.cout == 128) launch_tile<96, 128, 2, 8, 8>(s, A, pooled);  // e5: 8 channel groups
     else return false;
     return true;
 }
@@ -1156,10 +1159,12 @@ hipError_t forward(rc_ctx *ctx, Net &n, int N, int H, int W, int mirW = 0, int m
             A.mirW = mirW;
             A.mirH = mirH;
         }
-        // the encoders of the three finest levels also write their pooled output
-        A.pool = L <= 2 ? n.pooled[L] : nullptr;
-        // the three finest levels stage their input tiles in LDS
-        if (ctx->netKernels == RC_NET_KERNELS_TILED && sp.level <= 2 && launch_conv_tiled(s, A, pooledBy[L])) continue;
+        // the encoders write their pooled output (the next level's input) when
+        // their kernel can
+        A.pool = L <= 3 ? n.pooled[L] : nullptr;
+        // every layer of 128 input channels or fewer stages its input tiles in LDS
+        if (ctx->netKernels == RC_NET_KERNELS_TILED && launch_conv_tiled(s, A, pooledBy[L])) continue;
+        A.pool = nullptr;
         hipError_t e = launch_conv(s, A, ctx->netKernels != RC_NET_KERNELS_SINGLE_GROUP);
         if (e != hipSuccess) return e;
     }
