@@ -259,6 +259,12 @@ struct SweepSource {
 // previous output), with the display encode of the result into rgba8 (may be
 // null). *outFinal receives the buffer holding the chain output, *launches the
 // number of sweeps (rc_post.hip).
+// Workgroups of kBlock threads of `kernel` resident at once on the current
+// device, and a persistent launch's grid: that many, but no more than `units`
+// work items need (rc_pathtrace_lanes.hip).
+int resident_blocks(const void *kernel);
+int persistent_blocks(const void *kernel, int units);
+
 hipError_t launch_post_sweeps(hipStream_t stream, const SweepSource &src, const int *ops, const int *steps, int nOps,
                               int w, int h, float4 *bufA, float4 *bufB, uchar4 *rgba8, const rc_post_params &p,
                               const rc_temporal &t, const rc_temporal *dt, const ushort4 *gbuf,

@@ -700,6 +700,8 @@ __global__ void fill_int(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+}  // namespace
+
 // Resident workgroups per CU x CUs of a kernel on the current device, queried
 // once per (kernel, device): the query costs host time that a short launch
 // would otherwise wait for.
@@ -726,8 +728,6 @@ int persistent_blocks(const void *kernel, int units) {
     if (grid > need) grid = need;
     return grid < 1 ? 1 : (int)grid;
 }
-
-}  // namespace
 
 bool lanes_supported(const DeviceScene &D) { return D.valid && D.maxDepth < 30; }
 

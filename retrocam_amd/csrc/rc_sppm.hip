@@ -33,16 +33,16 @@ struct TraceArgs {
     int *heads, *next;
     unsigned int *counter;
     unsigned int *occupied;  // one bit per cell: set when a photon is stored in it
+    int *work;               // photon_trace_lanes: next photon index to take
 };
 
-// photon_trace.comp:168-251, one work-item per photon.
-__global__ void __launch_bounds__(kBlock) photon_trace_kernel(TraceArgs A) {
-    __shared__ int2 stk[kShortStack][kBlock];
-    const StackRef st{stk, (int)threadIdx.x};
+// Photon gid leaving the light (photon_trace.comp:168-200): light choice by
+// the power CDF, a uniform point on the light triangle and a cosine-weighted
+// direction, all drawn from the photon's own seed.
+__device__ __forceinline__ void emit_photon(const TraceArgs &A, uint32_t gid, uint32_t &seed, v3 &ro, v3 &rd,
+                                            v3 &power) {
     const SceneView &S = A.S;
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if ((int)gid >= A.photonCount || S.nLights == 0) return;
-    uint32_t seed = gid * 1664525u + (uint32_t)A.seed * 1013904223u + 2531011u;
+    seed = gid * 1664525u + (uint32_t)A.seed * 1013904223u + 2531011u;
     pcg(seed);
     pcg(seed);
     float u = rnd(seed);
@@ -64,68 +64,118 @@ __global__ void __launch_bounds__(kBlock) photon_trace_kernel(TraceArgs A) {
     v3 up = fabsf(en.y) < 0.9999f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
     v3 tt = normalize(cross(up, en));
     v3 bb = cross(en, tt);
-    v3 rd = normalize((tt * loc.x + bb * loc.y) + en * loc.z);
+    rd = normalize((tt * loc.x + bb * loc.y) + en * loc.z);
     v3 em = xyz(L3);
     float lum = (0.299f * em.x + 0.587f * em.y) + 0.114f * em.z;
     float pp = S.totalLightPower / (float)A.photonCount;
-    v3 power = lum > 1e-6f ? mk((em.x / lum) * pp, (em.y / lum) * pp, (em.z / lum) * pp) : mk(pp, pp, pp);
-    v3 ro = sp + en * 1e-3f;
-    bool spec = false;
-    for (int bounce = 0; bounce < 8; ++bounce) {
-        Hit h = closest_hit_unordered(S, ro, rd, st);
-        if (h.tri < 0) break;
-        const float4 *sh = S.triShade + 3 * (size_t)h.tri;
-        float4 na = sh[0], nb = sh[1], nc = sh[2];
-        float w = (1.0f - h.bu) - h.bv;
-        v3 hn = normalize((xyz(na) * w + xyz(nb) * h.bu) + xyz(nc) * h.bv);
-        v3 hp = ro + rd * h.t;
-        const float4 *M = S.mats + 4 * (size_t)rc_f2i_bits(na.w);
-        float4 m0 = M[0], m2 = M[2];
-        v3 n = dot(rd, hn) < 0.0f ? hn : neg(hn);
-        if (m2.z > 0.5f) {
-            bool entering = dot(rd, hn) < 0.0f;
-            float ior = m2.y;
-            float eta = entering ? (1.0f / ior) : ior;
-            float cosI = fabsf(dot(rd, n));
-            float r0 = (1.0f - ior) / (1.0f + ior);
-            r0 = r0 * r0;
-            float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
-            if (rnd(seed) < F) {
-                rd = reflect(rd, n);
-            } else {
-                v3 rf = refract(rd, n, eta);
-                if (dot(rf, rf) < 0.001f) rf = reflect(rd, n);
-                rd = normalize(rf);
-            }
-            ro = hp + rd * 2e-3f;
-            spec = true;
-            continue;
-        }
-        if (m0.w > 0.5f) {
-            power = power * xyz(m0);
+    power = lum > 1e-6f ? mk((em.x / lum) * pp, (em.y / lum) * pp, (em.z / lum) * pp) : mk(pp, pp, pp);
+    ro = sp + en * 1e-3f;
+}
+
+// One segment of a photon's path (photon_trace.comp:201-251): the closest hit,
+// then glass (Fresnel-chosen reflection or refraction) or metal (reflection,
+// power times albedo) continue the path; a diffuse hit after a specular bounce
+// stores the photon (storePhoton, :152-165) and ends it, as does a miss. Returns
+// whether the path continues.
+__device__ __forceinline__ bool photon_segment(const TraceArgs &A, const StackRef &st, uint32_t &seed, v3 &ro, v3 &rd,
+                                               v3 &power, bool &spec) {
+    const SceneView &S = A.S;
+    Hit h = closest_hit_unordered(S, ro, rd, st);
+    if (h.tri < 0) return false;
+    const float4 *sh = S.triShade + 3 * (size_t)h.tri;
+    float4 na = sh[0], nb = sh[1], nc = sh[2];
+    float w = (1.0f - h.bu) - h.bv;
+    v3 hn = normalize((xyz(na) * w + xyz(nb) * h.bu) + xyz(nc) * h.bv);
+    v3 hp = ro + rd * h.t;
+    const float4 *M = S.mats + 4 * (size_t)rc_f2i_bits(na.w);
+    float4 m0 = M[0], m2 = M[2];
+    v3 n = dot(rd, hn) < 0.0f ? hn : neg(hn);
+    if (m2.z > 0.5f) {
+        bool entering = dot(rd, hn) < 0.0f;
+        float ior = m2.y;
+        float eta = entering ? (1.0f / ior) : ior;
+        float cosI = fabsf(dot(rd, n));
+        float r0 = (1.0f - ior) / (1.0f + ior);
+        r0 = r0 * r0;
+        float F = r0 + (1.0f - r0) * rc_powf(rc_clampf(1.0f - cosI, 0.0f, 1.0f), 5.0f);
+        if (rnd(seed) < F) {
             rd = reflect(rd, n);
-            ro = hp + n * 2e-3f;
-            spec = true;
-            continue;
+        } else {
+            v3 rf = refract(rd, n, eta);
+            if (dot(rf, rf) < 0.001f) rf = reflect(rd, n);
+            rd = normalize(rf);
         }
-        if (spec) {  // storePhoton (photon_trace.comp:152-165)
-            unsigned int idx = atomicAdd(A.counter, 1u);
-            if (idx < (unsigned int)A.photonCount) {
-                int cx = (int)floorf(hp.x / A.radius), cy = (int)floorf(hp.y / A.radius),
-                    cz = (int)floorf(hp.z / A.radius);
-                const int h = cell_hash(cx, cy, cz, A.hashSize);
-                int old = atomicExch(&A.heads[h], (int)idx);
-                atomicOr(&A.occupied[h >> 5], 1u << (h & 31));
-                A.next[idx] = old;
-                float4 *rec = A.photons + 3 * (size_t)idx;
-                rec[0] = make_float4(hp.x, hp.y, hp.z, 0.0f);
-                rec[1] = make_float4(power.x, power.y, power.z, 0.0f);
-                rec[2] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+        ro = hp + rd * 2e-3f;
+        spec = true;
+        return true;
+    }
+    if (m0.w > 0.5f) {
+        power = power * xyz(m0);
+        rd = reflect(rd, n);
+        ro = hp + n * 2e-3f;
+        spec = true;
+        return true;
+    }
+    if (spec) {  // storePhoton (photon_trace.comp:152-165)
+        unsigned int idx = atomicAdd(A.counter, 1u);
+        if (idx < (unsigned int)A.photonCount) {
+            int cx = (int)floorf(hp.x / A.radius), cy = (int)floorf(hp.y / A.radius), cz = (int)floorf(hp.z / A.radius);
+            const int hc = cell_hash(cx, cy, cz, A.hashSize);
+            int old = atomicExch(&A.heads[hc], (int)idx);
+            atomicOr(&A.occupied[hc >> 5], 1u << (hc & 31));
+            A.next[idx] = old;
+            float4 *rec = A.photons + 3 * (size_t)idx;
+            rec[0] = make_float4(hp.x, hp.y, hp.z, 0.0f);
+            rec[1] = make_float4(power.x, power.y, power.z, 0.0f);
+            rec[2] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+        }
+    }
+    return false;
+}
+
+// The photons taken a segment at a time by persistent lanes: a lane whose path
+// ended takes the next photon index (one atomic per wave and round), so a wave
+// never waits for its longest path (photons that enter glass bounce several
+// times, most end at their first diffuse hit). Each photon's draws come from its
+// own seed, so the stored photons are those of one work-item per photon (the
+// reference's dispatch); only the order of insertion, an atomic order in both,
+// can differ.
+__global__ void __launch_bounds__(kBlock) photon_trace_lanes(TraceArgs A) {
+    __shared__ int2 stk[kShortStack][kBlock];
+    const StackRef st{stk, (int)threadIdx.x};
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    bool alive = false, done = false, spec = false;
+    int bounce = 0;
+    uint32_t seed = 0;
+    v3 ro = mk(0.0f, 0.0f, 0.0f), rd = mk(0.0f, 0.0f, 1.0f), power = ro;
+    while (true) {
+        const unsigned long long need = __ballot(!alive && !done);
+        if (need != 0ull) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(A.work, __popcll(need));
+            base = __shfl(base, leader, 64);
+            if (!alive && !done) {
+                const int gid = base + __popcll(need & below);
+                if (gid < A.photonCount) {
+                    emit_photon(A, (uint32_t)gid, seed, ro, rd, power);
+                    bounce = 0;
+                    spec = false;
+                    alive = true;
+                } else {
+                    done = true;
+                }
             }
         }
-        break;
+        if (__ballot(alive) == 0ull) break;
+        if (alive) {
+            const bool more = photon_segment(A, st, seed, ro, rd, power, spec);
+            alive = more && ++bounce < 8;
+        }
     }
 }
+
 
 struct GatherArgs {
     SceneView S;
@@ -304,7 +354,7 @@ int rc_sppm_trace(rc_ctx *ctx, int32_t photonCount, int32_t hashSize) {
     Sppm &s = ctx->sppm;
     // PhotonMap.clear (PhotonMap.java:84-106): heads := -1, counter := 0 (fill kernels)
     hipLaunchKernelGGL(fill_i32_kernel, dim3((hashSize + 255) / 256), dim3(256), 0, ctx->stream, s.heads, hashSize, -1);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, ctx->stream, (int *)s.counter, 1, 0);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, ctx->stream, (int *)s.counter, 2, 0);  // + work index
     const int words = (hashSize + 31) / 32;
     hipLaunchKernelGGL(fill_i32_kernel, dim3((words + 255) / 256), dim3(256), 0, ctx->stream, (int *)s.occupied, words,
                        0);
@@ -322,8 +372,9 @@ int rc_sppm_trace(rc_ctx *ctx, int32_t photonCount, int32_t hashSize) {
         A.next = s.next;
         A.counter = s.counter;
         A.occupied = s.occupied;
-        hipLaunchKernelGGL(photon_trace_kernel, dim3((photonCount + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           ctx->stream, A);
+        A.work = (int *)s.counter + 1;
+        const int grid = persistent_blocks((const void *)photon_trace_lanes, photonCount);
+        hipLaunchKernelGGL(photon_trace_lanes, dim3(grid), dim3(kBlock), 0, ctx->stream, A);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "photon trace launch");
