@@ -455,7 +455,7 @@ typedef struct rc_tuning {
                                (default 7: 4 waves/SIMD, 20 entries, one triangle) */
     int32_t laneBufferMiB;  /* RC_PT_LANES sample buffer bound (16 B per pixel and
                                sample; larger launches run in chunks), 64..65536,
-                               default 4096                                        */
+                               default 24576 (1.6 G pixel-samples per launch)      */
     int32_t laneTileWidth;  /* RC_PT_LANES unit tile: laneTileWidth x (64 /
                                laneTileWidth) pixels; 4, 8 (default), 16 or 32     */
     int32_t ptMode;         /* path-trace form (rc_set_pt_mode); rc_set_tuning

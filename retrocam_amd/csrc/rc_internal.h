@@ -149,8 +149,11 @@ struct rc_ctx {
     int32_t laneThresh = 32;  // lanes waiting before a wave shades (rc_pathtrace_lanes.hip)
     int32_t laneVariant = 7;  // lane-kernel variant (rc_pathtrace_lanes.hip)
     // Upper bound of the lane scheduler's sample buffer (16 B per pixel and
-    // sample): a launch with more samples x pixels runs in chunks.
-    int32_t laneBufferMiB = 4096;
+    // sample): a launch with more samples x pixels runs in chunks. Every chunk
+    // ends with each wave draining its lanes once per view, so a batch of views
+    // runs best as one chunk: 24 GiB holds 1.6 G pixel-samples (the buffer
+    // grows only to what a launch needs).
+    int32_t laneBufferMiB = 24576;
     int32_t laneTileShift = 3;  // unit tiles of 8 x 8 pixels
     rc::Net *net = nullptr;
     int32_t denoiser = RC_DENOISE_ATROUS;
