@@ -370,14 +370,14 @@ def read_valu(pt_mode, tuning):
 
 # U-Net methods of the quality sweep: one pass, or the mean of 2 / 4 passes over
 # the frame and its mirror images (rc_set_net_flips)
-NET_METHODS_PASSES = {"net": 1, "net2": 2, "net4": 4}
+NET_METHODS_PASSES = {"net": 1, "net2": 2, "net4": 4, "net8": 8}
 NET_METHODS = list(NET_METHODS_PASSES)
 
 
 def denoiser_name(method):
     if method in NET_METHODS_PASSES:
         k = NET_METHODS_PASSES[method]
-        return "U-Net" if k == 1 else f"U-Net ({k} mirrored passes)"
+        return "U-Net" if k == 1 else f"U-Net ({k} mirrored{' and moved' if k > 4 else ''} passes)"
     return "a-trous x5" if method == "atrous" else "none"
 
 
@@ -392,8 +392,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fixed-spp", type=int, default=0,
                     help="time frames at this spp (with --denoiser) instead of the U-Net's operating point")
-    ap.add_argument("--denoiser", choices=["atrous", "net", "net2", "net4"], default="atrous",
-                    help="denoiser of --fixed-spp frames (netK: U-Net averaged over K mirrored passes)")
+    ap.add_argument("--denoiser", choices=["atrous", "net", "net2", "net4", "net8"], default="atrous",
+                    help="denoiser of --fixed-spp frames (netK: U-Net averaged over K mirrored (K = 8: and moved) passes)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the quality reference (0 = skip)")
     ap.add_argument("--sweep", default="4,5,6,7,8,10,12,14,16,20,24,32,40,48,56,64,80,96,112,128,192,256,512,1024",
                     help="spp values searched for the relMSE <= 0.01 operating points")

@@ -388,10 +388,11 @@ int rc_net_grad(rc_ctx *ctx, const float *raw, const float *target, int32_t n, i
 int rc_net_forward(rc_ctx *ctx, const float *raw, int32_t n, int32_t h, int32_t w, float *out);
 /* Which denoiser rc_post runs when rc_post_params.atrousEnabled is set. */
 int rc_set_denoiser(rc_ctx *ctx, int32_t mode);
-/* Network passes per denoise (1, 2 or 4; default 1): the frame as is, then
- * mirrored left-right, top-bottom and both; the denoised image is the mean of
- * the passes' outputs mirrored back (a flip ensemble: lower error per sample
- * for more network time). Applies to rc_post and rc_views_post. */
+/* Network passes per denoise (1, 2, 4 or 8; default 1): the frame as is, then
+ * mirrored left-right, top-bottom and both, then (8) those four moved 8 pixels
+ * right and down; the denoised image is the mean of the passes' outputs mapped
+ * back (an ensemble: lower error per sample for more network time). Applies to
+ * rc_post and rc_views_post. */
 int rc_set_net_flips(rc_ctx *ctx, int32_t passes);
 
 /* Convolution kernels of the network's forward pass (no counterpart in the

@@ -119,13 +119,25 @@ struct ConvArgs {
     uint16_t *pool;        // conv3x3_tile: also the 2 x 2 mean of the output (pool2), or null
     // mirW > 0: a holds ONE image (N = 1 layout) of a w x h frame padded to W x H
     // by edge replication; image n of the batch is that frame mirrored by n (bit
-    // 0 left-right, bit 1 top-bottom), read in place
+    // 0 left-right, bit 1 top-bottom) and, for bit 2, moved by kEnsembleShift
+    // pixels right and down, read in place
     int mirW, mirH;
 };
 
+// The diagonal move of the ensemble's images 4..7: half the 16-pixel period of
+// the network's coarsest level, so that their pooling windows straddle those of
+// images 0..3.
+constexpr int kEnsembleShift = 8;
+
 // Pixel of the single stored image that image n's pixel (y, x) reads when the
-// batch is the mirror ensemble of one padded w x h frame (ConvArgs::mirW).
+// batch is the ensemble of one padded w x h frame (ConvArgs::mirW): the frame's
+// pixel (x, y), or (x - s, y - s) for images 4..7, clamped to the frame (edge
+// replication), then mirrored.
 __device__ __forceinline__ size_t mirror_src(int n, int y, int x, int W, int w, int h) {
+    if (n & 4) {
+        x = max(x - kEnsembleShift, 0);
+        y = max(y - kEnsembleShift, 0);
+    }
     int xb = min(x, w - 1), yb = min(y, h - 1);
     if (n & 1) xb = w - 1 - xb;
     if (n & 2) yb = h - 1 - yb;
@@ -683,10 +695,11 @@ __device__ __forceinline__ float head_channel(const uint16_t *wb, const float *b
 }
 
 // Head of a flip ensemble: the batch holds `passes` images of (Hp, Wp), image k
-// the frame mirrored by k (bit 0 left-right, bit 1 top-bottom); the features are
-// the frame's, stored once. Pixel (x, y) of
-// the w x h output is ((v0 + v1) + v2 + v3) * (1 / passes) of the clamped head
-// outputs vk at the mirrored positions; one pass writes v0 itself.
+// the frame mirrored by k (bit 0 left-right, bit 1 top-bottom) and moved by
+// kEnsembleShift pixels right and down for bit 2; the features are the frame's,
+// stored once. Pixel (x, y) of the w x h output is (((v0 + v1) + v2) + ...) *
+// (1 / passes) of the clamped head outputs vk at the pixel's positions in the
+// images; one pass writes v0 itself.
 struct EnsembleArgs {
     const uint16_t *d1, *feat, *wb;
     const float *bias;
@@ -703,7 +716,8 @@ __global__ void __launch_bounds__(256) head_ensemble_kernel(EnsembleArgs A) {
     float acc[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
     for (int k = 0; k < A.passes; ++k) {
-        const int xk = (k & 1) ? A.w - 1 - x : x, yk = (k & 2) ? A.h - 1 - y : y;
+        const int sh = (k & 4) ? kEnsembleShift : 0;
+        const int xk = ((k & 1) ? A.w - 1 - x : x) + sh, yk = ((k & 2) ? A.h - 1 - y : y) + sh;
         const size_t p = (size_t)k * A.Wp * A.Hp + (size_t)yk * A.Wp + xk;
         float d1v[32];
 #pragma unroll
@@ -1369,13 +1383,16 @@ bool net_ready(const rc_ctx *ctx) { return ctx->net && ctx->net->ready; }
 
 // Denoises the post image src (w x h RGBA32F, p00 output) into dst using the
 // frame's G-buffers: the mean of ctx->netFlips network passes over the frame
-// as is, mirrored left-right, top-bottom and both (in that order), each pass's
-// output mirrored back; ((p0 + p1) + p2 + p3) * (1 / passes) per channel. The
-// passes run as one batch of images through the network.
+// as is, mirrored left-right, top-bottom and both (in that order), then (8
+// passes) the same four moved kEnsembleShift pixels right and down, each pass's
+// output mapped back; (((p0 + p1) + p2) + ...) * (1 / passes) per channel. The
+// passes run as one batch of images through the network; with 8 passes the
+// padded image has room for the move.
 int run_net_denoise(rc_ctx *ctx, const float4 *src, const ushort4 *gbuf, const ushort4 *galbedo,
                     const float *variance, int total, float4 *dst, int w, int h) {
     Net &n = *ctx->net;
-    const int Wp = (w + 15) / 16 * 16, Hp = (h + 15) / 16 * 16;
+    const int room = ctx->netFlips > 4 ? kEnsembleShift : 0;
+    const int Wp = (w + room + 15) / 16 * 16, Hp = (h + room + 15) / 16 * 16;
     const int passes = ctx->netFlips;
     if ((size_t)passes * Wp * Hp > (size_t)64 << 20)
         return set_error(ctx, RC_ERR_LIMIT, "net denoise: passes x padded frame above 64 Mpixels (lower rc_set_net_flips)");
@@ -1624,8 +1641,8 @@ int rc_set_net_kernels(rc_ctx *ctx, int32_t mode) {
 
 int rc_set_net_flips(rc_ctx *ctx, int32_t passes) {
     if (!ctx) return RC_ERR_INVALID_ARG;
-    if (passes != 1 && passes != 2 && passes != 4)
-        return set_error(ctx, RC_ERR_INVALID_ARG, "passes must be 1, 2 or 4");
+    if (passes != 1 && passes != 2 && passes != 4 && passes != 8)
+        return set_error(ctx, RC_ERR_INVALID_ARG, "passes must be 1, 2, 4 or 8");
     ctx->netFlips = passes;
     return RC_OK;
 }

@@ -92,8 +92,9 @@ class NetDenoiser:
         return out
 
     def set_flips(self, passes: int = 1):
-        """rc_set_net_flips: network passes per denoise (1, 2 or 4), averaged over
-        the frame and its mirror images."""
+        """rc_set_net_flips: network passes per denoise (1, 2, 4 or 8), averaged
+        over the frame, its mirror images and (8) those moved 8 pixels right and
+        down."""
         self.ctx.check(self._lib.rc_set_net_flips(self.ctx.h, int(passes)))
 
     def set_kernels(self, mode: int = N.RC_NET_KERNELS_TILED):

@@ -122,12 +122,14 @@ def test_post_chain_runs_the_network(gpu_ctx):
     assert rel_mse(out.hdr[:, :3], ref) <= 1e-6
 
 
-@pytest.mark.parametrize("passes", [2, 4])
+@pytest.mark.parametrize("passes", [2, 4, 8])
 def test_flip_ensemble_is_the_mean_of_mirrored_passes(gpu_ctx, passes):
     # rc_set_net_flips: pass k runs on the frame mirrored by k (bit 0 left-right,
-    # bit 1 top-bottom); its output is mirrored back and the passes averaged as
-    # ((p0 + p1) + p2 + p3) * (1 / passes). The same composition of rc_net_forward
-    # calls on the host must give the post chain's image bit for bit.
+    # bit 1 top-bottom) and, for k >= 4, moved 8 pixels right and down (edge
+    # replication fills the top rows and left columns); its output is mapped
+    # back and the passes averaged as (((p0 + p1) + p2) + ...) * (1 / passes).
+    # The same composition of rc_net_forward calls on the host must give the
+    # post chain's image bit for bit.
     w = workloads.training_scene(4, size=64)
     W, H = 90, 52  # not multiples of 16: the frame is padded by edge replication
     cam = CameraSpec(**{**w.camera.__dict__, "width": W, "height": H})
@@ -144,16 +146,18 @@ def test_flip_ensemble_is_the_mean_of_mirrored_passes(gpu_ctx, passes):
         net.set_flips(1)
         net.enable(False)
     raw = frame_raw(r, W, H)
-    Wp, Hp = 96, 64
+    room = 8 if passes > 4 else 0  # the padded frame leaves room for the move
+    Wp, Hp = (W + room + 15) // 16 * 16, (H + room + 15) // 16 * 16
 
-    def fwd(x):
-        xp = np.pad(x, ((0, Hp - H), (0, Wp - W), (0, 0)), mode="edge")
-        return np.maximum(net.forward(xp[None])[0][:H, :W], np.float32(0))
+    def fwd(x, s):
+        xp = np.pad(x, ((s, Hp - H - s), (s, Wp - W - s), (0, 0)), mode="edge")
+        return np.maximum(net.forward(xp[None])[0][s:s + H, s:s + W], np.float32(0))
 
-    flips = [lambda a: a, lambda a: a[:, ::-1], lambda a: a[::-1], lambda a: a[::-1, ::-1]][:passes]
+    flips = [lambda a: a, lambda a: a[:, ::-1], lambda a: a[::-1], lambda a: a[::-1, ::-1]]
     acc = None
-    for f in flips:
-        y = f(fwd(f(raw)))
+    for k in range(passes):
+        f, s = flips[k & 3], 8 if k & 4 else 0
+        y = f(fwd(f(raw), s))
         acc = y if acc is None else (acc + y).astype(np.float32)
     ref = (acc * np.float32(1.0 / passes)).astype(np.float32).reshape(-1, 3)
     assert np.array_equal(out.hdr[:, :3], ref)
