@@ -505,7 +505,7 @@ def main():
             ok = [s for s, e in curves[m] if e <= 0.01]
             if ok:
                 s_m = min(ok)
-                ms = time_frames(s_m, m, 3)
+                ms = time_frames(s_m, m, 10 if s_m <= 64 else 3)
                 op[m] = {"spp": s_m, "relmse": dict(curves[m])[s_m], "frames_per_s": round(1e3 / ms, 4)}
         quality["operating_points"] = op
         best_net = max((m for m in NET_METHODS if m in op), key=lambda m: op[m]["frames_per_s"], default=None)

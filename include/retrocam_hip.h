@@ -453,7 +453,8 @@ typedef struct rc_tuning {
                                many of its 64 lanes wait (1..64, default 32)       */
     int32_t laneVariant;    /* RC_PT_LANES kernel variant, 0..RC_LANE_VARIANTS-1:
                                occupancy, LDS stack depth, triangles per step
-                               (default 7: 4 waves/SIMD, 20 entries, one triangle) */
+                               (default 9: 4 waves/SIMD, 20 entries, a triangle
+                               and a node per walk step)                           */
     int32_t laneBufferMiB;  /* RC_PT_LANES sample buffer bound (16 B per pixel and
                                sample; larger launches run in chunks), 64..65536,
                                default 24576 (1.6 G pixel-samples per launch)      */

@@ -147,7 +147,7 @@ struct rc_ctx {
     int32_t ptMode = RC_PT_LANES;
     rc::LaneBuffers lanes;
     int32_t laneThresh = 32;  // lanes waiting before a wave shades (rc_pathtrace_lanes.hip)
-    int32_t laneVariant = 7;  // lane-kernel variant (rc_pathtrace_lanes.hip)
+    int32_t laneVariant = 9;  // lane-kernel variant (rc_pathtrace_lanes.hip)
     // Upper bound of the lane scheduler's sample buffer (16 B per pixel and
     // sample): a launch with more samples x pixels runs in chunks. Every chunk
     // ends with each wave draining its lanes once per view, so a batch of views

@@ -535,12 +535,13 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
 // one node (2, walk_step_dual):
 //   0: W 7, NS 11            1: W 6, NS 13  2: W 8, NS 10  3: W 5, NS 16
 //   4: W 7, NS 11, TRI       5: W 6, NS 13, TRI       6: W 5, NS 16, TRI
-//   7: W 4, NS 20, TRI (default)                      8: W 3, NS 26, TRI
-//   9: W 4, NS 20, dual     10: W 5, NS 16, dual
+//   7: W 4, NS 20, TRI                                8: W 3, NS 26, TRI
+//   9: W 4, NS 20, dual (default)                    10: W 5, NS 16, dual
 // On the 250k-triangle frame at 80 samples: variant 0 177 ms, 4 141 ms, 6 138 ms,
 // 7 121 ms (no register spills; 117 VGPRs since the camera is read through the
 // constant address space; the 5-wave and higher variants spill around the shading
-// block and move about 5x the HBM bytes in scratch traffic), 8 144 ms.
+// block and move about 5x the HBM bytes in scratch traffic), 8 144 ms, 9 120 ms;
+// at 5 samples variant 9 (118 VGPRs) takes 8.39-8.45 ms against 8.48-8.56 for 7.
 template <bool STATS, int W, int NS, int WALK>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W, 8)))
 pathtrace_lanes_kernel(PassArgs A, float4 *samples, int *counter, int thresh, float4 *cont) {

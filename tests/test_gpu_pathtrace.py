@@ -181,7 +181,7 @@ def test_lane_tuning_changes_no_result(gpu_ctx, cornell):
         got = gpu_render(gpu_ctx, w.scene, cam, w.params, 4, batch=4, mode=N.RC_PT_LANES)
         for k in ref:
             assert np.array_equal(ref[k], got[k]), (tw, k)
-    r.set_tuning(laneThreshold=32, laneVariant=7, laneBufferMiB=4096, laneTileWidth=8, ptMode=N.RC_PT_LANES)
+    r.set_tuning(laneThreshold=32, laneVariant=9, laneBufferMiB=24576, laneTileWidth=8, ptMode=N.RC_PT_LANES)
     for bad in ({"laneThreshold": 0}, {"laneThreshold": 65}, {"laneVariant": N.RC_LANE_VARIANTS},
                 {"laneBufferMiB": 63}, {"laneTileWidth": 2}, {"laneTileWidth": 12}):
         with pytest.raises(RenderError) as e:
@@ -197,7 +197,7 @@ def test_lanes_chunked_launch(gpu_ctx, cornell):
     r = Renderer(gpu_ctx)
     r.set_tuning(laneBufferMiB=64)
     got = gpu_render(gpu_ctx, w.scene, w.camera, w.params, 100, batch=100, mode=N.RC_PT_LANES)
-    r.set_tuning(laneBufferMiB=4096)
+    r.set_tuning(laneBufferMiB=24576)
     for k in ref:
         assert np.array_equal(ref[k], got[k]), k
 
@@ -217,7 +217,7 @@ def test_autotune_keeps_a_valid_tuning(gpu_ctx, cornell):
     out = gpu_render(gpu_ctx, w.scene, cam, w.params, 3, batch=3, mode=N.RC_PT_LANES)
     for k in ref:
         assert np.array_equal(ref[k], out[k]), k
-    r.set_tuning(laneThreshold=32, laneVariant=7, laneTileWidth=8, ptMode=N.RC_PT_LANES)
+    r.set_tuning(laneThreshold=32, laneVariant=9, laneTileWidth=8, ptMode=N.RC_PT_LANES)
 
 
 def test_lanes_deep_bvh_falls_back(gpu_ctx):
