@@ -73,7 +73,8 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights (.npy) instead of a seeded init")
     ap.add_argument("--scene-offset", type=int, default=0, help="training views use seeds offset .. offset+scenes-1")
     ap.add_argument("--frame-offset", type=int, default=2,
-                    help="full-frame seeds offset .. offset+frame-seeds-1 (seed 1, the benchmark frame, is refused)")
+                    help="full-frame seeds offset .. offset+frame-seeds-1 (seed 1, the benchmark frame, and the "
+                         "held-out / validation frames 900..999 are refused)")
     ap.add_argument("--spp-in", default="4,8,16,32,64,128,256",
                     help="input sample counts the levels are drawn from (comma-separated)")
     args = ap.parse_args(argv)
@@ -88,6 +89,8 @@ def main(argv=None):
         raise SystemExit("--spp-in needs at least --levels values")
     if args.frame_offset <= 1 < args.frame_offset + args.frame_seeds:
         raise SystemExit("full-frame seed 1 is the benchmark frame: use --frame-offset >= 2")
+    if args.frame_seeds > 0 and args.frame_offset < 1000 and args.frame_offset + args.frame_seeds > 900:
+        raise SystemExit("full-frame seeds 900..999 are the held-out and validation frames: use other seeds")
     scene_seeds = range(args.scene_offset, args.scene_offset + args.scenes)
     if any(s >= 1_000_000 for s in (scene_seeds[-1:] if args.scenes else [])):
         raise SystemExit("training view seeds >= 1000000 are the held-out evaluation views")
