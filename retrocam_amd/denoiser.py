@@ -153,13 +153,27 @@ def _raw_from(acc, gbuf, galb, variance, w, h):
     return raw.reshape(h, w, RAW_CHANNELS)
 
 
+def _raw_between(a0, v0, a1, v1, gbuf, galb, w, h):
+    """Raw inputs of the samples added between two reads of one FP64
+    accumulation (sums a0 -> a1 and luma^2 sums v0 -> v1): the mean of those
+    samples alone, an input independent of the earlier ones."""
+    n = a1[:, 3] - a0[:, 3]
+    acc = np.empty((a1.shape[0], 4), np.float64)
+    acc[:, 0:3] = a1[:, 0:3] - a0[:, 0:3]
+    acc[:, 3] = n
+    var = v1.astype(np.float64) - v0.astype(np.float64)
+    return _raw_from(acc, gbuf, galb, var, w, h)
+
+
 def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256), spp_ref: int = 1024, log=None,
-                 group: int = N.RC_MAX_SCENE_SLOTS, levels: int = 1, seed: int = 1):
+                 group: int = N.RC_MAX_SCENE_SLOTS, levels: int = 1, seed: int = 1, independent: bool = False):
     """Pairs over workloads.training_scene(seed); spp_in cycles through the given
     counts. With levels > 1 each view yields `levels` inputs at increasing sample
-    counts drawn from spp_in (one accumulation read at each count) that share
-    its high-spp target. Views are rendered in batches of `group` scenes, one
-    persistent launch per batch and sample chunk (rc_views_render)."""
+    counts drawn from spp_in that share its high-spp target: reads of one
+    accumulation at each count, or, with `independent`, the means of disjoint
+    runs of samples (frames n1.., n1 + n2.., ...: independent noise at each
+    level). Views are rendered in batches of `group` scenes, one persistent
+    launch per batch and sample chunk (rc_views_render)."""
     from . import workloads
     from .batch import ViewBatch, ViewRequest
     seeds = list(seeds)
@@ -176,7 +190,18 @@ def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256
             lv = [sorted(int(v) for v in rng.choice(spp_in, size=levels, replace=False)) for _ in ws]
         done = [0] * len(ws)
         per_view = [[] for _ in ws]
+        prev = [None] * len(ws)
         for r in range(len(lv[0])):
+            if independent:
+                reqs = [ViewRequest(k, w.camera, w.params, lv[k][r]) for k, w in enumerate(ws)]
+                vb.render(reqs, reset=r == 0, fp64=True)
+                for k in range(len(ws)):
+                    a1, v1 = vb.read(k, N.RC_BUF_ACCUM64), vb.read(k, N.RC_BUF_VARIANCE)
+                    a0, v0 = prev[k] if prev[k] is not None else (np.zeros_like(a1), np.zeros_like(v1))
+                    per_view[k].append(_raw_between(a0, v0, a1, v1, vb.read(k, N.RC_BUF_GBUFFER),
+                                                    vb.read(k, N.RC_BUF_GALBEDO), size, size))
+                    prev[k] = (a1, v1)
+                continue
             reqs = [ViewRequest(k, w.camera, w.params, lv[k][r] - done[k]) for k, w in enumerate(ws)]
             vb.render(reqs, reset=r == 0)
             for k in range(len(ws)):
@@ -203,10 +228,11 @@ def make_dataset(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256
 
 
 def make_frame_crops(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128, 256), spp_ref: int = 1024,
-                     levels: int = 3, log=None, seed: int = 2):
+                     levels: int = 3, log=None, seed: int = 2, independent: bool = False):
     """Pairs from full frames of workloads.full_frame(seed) (854x480, 250k
     triangles, thin lens), cut into size x size crops: `levels` inputs per frame
-    at increasing sample counts from spp_in, targets at spp_ref with FP64
+    at increasing sample counts from spp_in (with `independent`, means of
+    disjoint runs of samples, as in make_dataset), targets at spp_ref with FP64
     accumulation."""
     from . import workloads
     from .renderer import Renderer, SceneUploader
@@ -218,15 +244,27 @@ def make_frame_crops(ctx, seeds, size: int = 128, spp_in=(4, 8, 16, 32, 64, 128,
         W, H = w.camera.width, w.camera.height
         SceneUploader(ctx).upload(w.scene)
         lv = sorted(int(v) for v in rng.choice(spp_in, size=levels, replace=False))
-        r.set_accum_mode(N.RC_ACCUM_FP32)
-        r.reset()
         full = []
-        done = 0
-        for n in lv:
-            r.render(w.camera, w.params, n_samples=n - done)
-            done = n
-            full.append(_raw_from(r.read(N.RC_BUF_ACCUM), r.read(N.RC_BUF_GBUFFER), r.read(N.RC_BUF_GALBEDO),
-                                  r.read(N.RC_BUF_VARIANCE), W, H))
+        if independent:
+            r.set_accum_mode(N.RC_ACCUM_FP64)
+            r.reset()
+            a0, v0 = None, None
+            for n in lv:
+                r.render(w.camera, w.params, n_samples=n)
+                a1, v1 = r.read(N.RC_BUF_ACCUM64), r.read(N.RC_BUF_VARIANCE)
+                if a0 is None:
+                    a0, v0 = np.zeros_like(a1), np.zeros_like(v1)
+                full.append(_raw_between(a0, v0, a1, v1, r.read(N.RC_BUF_GBUFFER), r.read(N.RC_BUF_GALBEDO), W, H))
+                a0, v0 = a1, v1
+        else:
+            r.set_accum_mode(N.RC_ACCUM_FP32)
+            r.reset()
+            done = 0
+            for n in lv:
+                r.render(w.camera, w.params, n_samples=n - done)
+                done = n
+                full.append(_raw_from(r.read(N.RC_BUF_ACCUM), r.read(N.RC_BUF_GBUFFER), r.read(N.RC_BUF_GALBEDO),
+                                      r.read(N.RC_BUF_VARIANCE), W, H))
         r.set_accum_mode(N.RC_ACCUM_FP64)
         r.reset()
         done = 0

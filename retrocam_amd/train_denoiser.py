@@ -77,6 +77,9 @@ def main(argv=None):
                          "held-out / validation frames 900..999 are refused)")
     ap.add_argument("--spp-in", default="4,8,16,32,64,128,256",
                     help="input sample counts the levels are drawn from (comma-separated)")
+    ap.add_argument("--independent-levels", action="store_true",
+                    help="each input level from its own run of samples (independent noise) instead of reads "
+                         "of one accumulation")
     args = ap.parse_args(argv)
 
     def log(*a):
@@ -95,11 +98,12 @@ def main(argv=None):
     if any(s >= 1_000_000 for s in (scene_seeds[-1:] if args.scenes else [])):
         raise SystemExit("training view seeds >= 1000000 are the held-out evaluation views")
     raw, tgt = make_dataset(ctx, scene_seeds, size=args.size, spp_in=spp_in, spp_ref=args.ref_spp, log=log,
-                            levels=args.levels, seed=args.seed)
+                            levels=args.levels, seed=args.seed, independent=args.independent_levels)
     if args.frame_seeds > 0:
         fr, ft = make_frame_crops(ctx, range(args.frame_offset, args.frame_offset + args.frame_seeds), size=args.size,
                                   spp_in=spp_in,
-                                  spp_ref=args.ref_spp, levels=args.levels, log=log, seed=args.seed + 1)
+                                  spp_ref=args.ref_spp, levels=args.levels, log=log, seed=args.seed + 1,
+                                  independent=args.independent_levels)
         raw, tgt = np.concatenate([raw, fr]), np.concatenate([tgt, ft])
         del fr, ft
     t_data = time.perf_counter() - t0
@@ -127,6 +131,7 @@ def main(argv=None):
     summary = {
         "init": args.init, "scene_offset": args.scene_offset, "frame_offset": args.frame_offset,
         "scenes": args.scenes, "levels": args.levels, "spp_in": list(spp_in), "frame_seeds": args.frame_seeds,
+        "independent_levels": bool(args.independent_levels),
         "pairs": int(raw.shape[0]),
         "size": args.size, "ref_spp": args.ref_spp, "steps": args.steps,
         "batch": args.batch, "lr": args.lr, "params": net.n,

@@ -95,3 +95,28 @@ def test_features_closed_forms():
     assert np.allclose(f[9], 0.25)
     assert np.isclose(f[10], np.log1p(2.0))
     assert np.isclose(f[11], 0.4)
+
+
+def test_independent_level_inputs_are_segment_means():
+    """Inputs taken as differences of two reads of one FP64 accumulation equal
+    the raw inputs of the later samples on their own (mean, mean luma^2, count)."""
+    from retrocam_amd.denoiser import _raw_between, _raw_from
+    rng = np.random.default_rng(5)
+    w, h, n0, n1 = 4, 2, 3, 7
+    s = rng.uniform(0, 4, (w * h, n1, 3))
+    l2 = ((s[..., 0] * 0.2126 + s[..., 1] * 0.7152) + s[..., 2] * 0.0722) ** 2
+    gbuf = rng.uniform(-1, 1, (w * h, 4)).astype(np.float16).view(np.uint16)
+    galb = rng.uniform(0, 1, (w * h, 4)).astype(np.float16).view(np.uint16)
+
+    def acc(a, b):
+        out = np.empty((w * h, 4))
+        out[:, 0:3] = s[:, a:b].sum(axis=1)
+        out[:, 3] = b - a
+        return out
+
+    got = _raw_between(acc(0, n0), l2[:, :n0].sum(1).astype(np.float32), acc(0, n1),
+                       l2[:, :n1].sum(1).astype(np.float32), gbuf, galb, w, h)
+    want = _raw_from(acc(n0, n1), gbuf, galb, l2[:, n0:n1].sum(1), w, h)
+    assert np.allclose(got[..., 11], n1 - n0)
+    assert np.allclose(got[..., 0:3], s[:, n0:n1].mean(1).reshape(h, w, 3), rtol=1e-6)
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-6)
