@@ -193,9 +193,9 @@ int plan_lanes(rc_ctx *ctx, ViewTables &T) {
     long long K = maxS;
     while (K > 1 && bytesFor(K) > cap) K = (K + 1) / 2;
     for (int k = 0; k < n; ++k)
-        while (K > 1 && tilePx[k] * K >= (1ll << 30)) K = (K + 1) / 2;
+        while (K > 1 && tilePx[k] * (K + 1) > (1ll << 30)) K = (K + 1) / 2;  // + the G-buffer rays
     for (int k = 0; k < n; ++k)
-        if (tilePx[k] * K >= (1ll << 30)) return set_error(ctx, RC_ERR_LIMIT, "views: view too large");
+        if (tilePx[k] * (K + 1) > (1ll << 30)) return set_error(ctx, RC_ERR_LIMIT, "views: view too large");
     T.nChunks = (int)((maxS + K - 1) / K);
     const long long entries = (long long)T.nChunks * n;
     if (entries > (1 << 24)) return set_error(ctx, RC_ERR_LIMIT, "views: too many chunks");
@@ -219,6 +219,7 @@ int plan_lanes(rc_ctx *ctx, ViewTables &T) {
             L.nSamples = (int)ns;
             L.nTilePx = (int)tilePx[k];
             T.units[c] += ns * tilePx[k];
+            if (ns > 0 && T.args[k].frame0 + s0 == 0) T.units[c] += tilePx[k];  // G-buffer rays
         }
         off += std::min<long long>(K, T.args[k].nSamples) * tilePx[k];
     }

@@ -16,7 +16,9 @@
 //     lanes_reduce then adds the samples of every pixel in frame order, which
 //     is exactly the reference's sequential running sum (pathtrace.comp:633-654).
 // Units are numbered tile-major (8x8 pixel tiles, then sample, then pixel in
-// tile), so the 64 units a wave takes at once are one tile at one sample.
+// tile), so the 64 units a wave takes at once are one tile at one sample. A
+// launch that holds frame 0 ends with one more unit per pixel, its G-buffer
+// camera ray, so that single rays fill the time in which the last paths finish.
 //
 // Shadow rays use the same ordered walk as closest hits, with the walk's
 // maximum t set to maxDist and an exit on the first accepted triangle. For a
@@ -308,7 +310,12 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
     const bool lca = (fabsf(cam.lca[0]) + fabsf(cam.lca[2])) > 1e-4f;
     const LaneTile tl(cam.width, cam.height, A.tileShift);
     const int nTilePx = tl.pixels();
-    const int nUnits = nTilePx * K;  // < 2^30, checked by the host
+    // Units: K path samples of every pixel, then, when the launch holds frame 0,
+    // one G-buffer camera ray per pixel (s == K). The G-buffer rays come last
+    // because they are single rays: while the last paths finish, the other lanes
+    // take them, so the launch's end is not the long paths' alone.
+    const int nPathUnits = nTilePx * K;
+    const int nUnits = nPathUnits + (frame0 == 0 ? nTilePx : 0);  // <= 2^30, checked by the host
     const unsigned long long below = (1ull << lane) - 1ull;
     const size_t nLanes = (size_t)gridDim.x * kBlock;
     float4 *const myCont = cont + (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -343,9 +350,15 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                     phase = PH_DONE;
                 } else {
                     const int within = u & 63;
-                    const int ts = u >> 6;  // tile * K + sample
-                    const int tile = ts / K;
-                    s = ts - tile * K;
+                    const int ts = u >> 6;  // tile * K + sample, or a G-buffer tile
+                    int tile;
+                    if (u < nPathUnits) {
+                        tile = ts / K;
+                        s = ts - tile * K;
+                    } else {
+                        tile = ts - (nPathUnits >> 6);
+                        s = K;
+                    }
                     int px, py;
                     tl.pixel(tile, within, px, py);
                     pxy = (uint32_t)px | ((uint32_t)py << 16);
@@ -368,7 +381,7 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
             if (phase == PH_START) {
                 ch = 0;
                 firstTri = -1;
-                if (frame == 0) {
+                if (s == K) {
                     // G-buffer from the seed-99991 camera ray (pathtrace.comp:590-595, 643-646)
                     uint32_t gs = init_seed(px, py, 99991);
                     gen_ray(cam, px, py, gs, 0.0f, w.ro, w.rd);
@@ -389,7 +402,7 @@ __device__ void lanes_view(const PassArgs &A, int frame0, int K, float4 *__restr
                 }
                 A.gbuf[pi] = make_ushort4(rc_f32_to_f16(g0), rc_f32_to_f16(g1), rc_f32_to_f16(g2), rc_f32_to_f16(g3));
                 A.gbufId[pi] = w.htri;
-                newPath = true;
+                phase = PH_FETCH;  // a G-buffer unit is this one ray
             } else if (kind == RAY_SHADOW) {
                 const float4 c0 = myCont[0], c1 = myCont[nLanes];
                 if (w.htri < 0) {
@@ -735,7 +748,7 @@ bool lanes_supported(const DeviceScene &D) { return D.valid && D.maxDepth < 30; 
 int lanes_chunk_samples(int width, int height, int tileShift, int nSamples, long long capBytes) {
     const long long tilePx = LaneTile(width, height, tileShift).pixels();
     long long k = capBytes / (tilePx * 16);
-    const long long kUnits = (1ll << 30) / tilePx;
+    const long long kUnits = (1ll << 30) / tilePx - 1;  // K samples + the G-buffer rays
     if (k > kUnits) k = kUnits;
     if (k < 1) k = 1;
     return (int)(k < nSamples ? k : nSamples);
@@ -808,7 +821,7 @@ hipError_t launch_lanes(rc_ctx *ctx, const PassArgs &A0, int thresh, int variant
         A.frame0 = A0.frame0 + s0;
         A.nSamples = A0.nSamples - s0 < kMax ? A0.nSamples - s0 : kMax;
         hipLaunchKernelGGL(fill_int, dim3(1), dim3(256), 0, ctx->stream, ctx->lanes.counters, 1, 0);
-        const int grid = persistent_blocks((const void *)kernel, nTilePx * A.nSamples);
+        const int grid = persistent_blocks((const void *)kernel, nTilePx * (A.nSamples + (A.frame0 == 0 ? 1 : 0)));
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, A, ctx->lanes.samples,
                            ctx->lanes.counters, thresh, ctx->lanes.cont);
         ReduceArgs R;
