@@ -118,11 +118,15 @@ __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int i
     const float4 *p = S.triPos + 3 * (size_t)idx;
     float4 a = p[0], b = p[1], c = p[2];
     v3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
+    v3 s = ro - v0;
+    // s is formed here, before the det test, so that v0 is loaded together with
+    // e1 and e2: left to itself the compiler moves v0's load below the det test,
+    // which costs every accepted determinant a second memory round trip.
+    asm volatile("" : "+v"(s.x), "+v"(s.y), "+v"(s.z));
     v3 h = cross(rd, e2);
     float det = dot(e1, h);
     if (fabsf(det) < 1e-8f) return false;
     float f = recip_rn(det);
-    v3 s = ro - v0;
     float u = f * dot(s, h);
     if (u < 0.0f || u > 1.0f) return false;
     v3 q = cross(s, e1);
