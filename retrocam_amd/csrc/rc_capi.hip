@@ -478,16 +478,16 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
         }
         return innerIdx[node];
     };
-    std::vector<float4> inner((size_t)nInner * 4);
+    // inner node records of 14 words (56 B, packed): both child boxes, then the refs
+    std::vector<float> inner((size_t)nInner * 14);
     for (int i = 0; i < nNodes; ++i) {
         if (PC[i] > 0) continue;
         const float *l = nodes + 12 * (size_t)L[i];
         const float *r = nodes + 12 * (size_t)R[i];
-        float4 *o = &inner[(size_t)innerIdx[i] * 4];
-        o[0] = make_float4(l[0], l[1], l[2], l[4]);
-        o[1] = make_float4(l[5], l[6], r[0], r[1]);
-        o[2] = make_float4(r[2], r[4], r[5], r[6]);
-        o[3] = make_float4(bits_f(refOf(L[i])), bits_f(refOf(R[i])), 0.0f, 0.0f);
+        float *o = &inner[(size_t)innerIdx[i] * 14];
+        const float rec[14] = {l[0], l[1], l[2], l[4], l[5], l[6], r[0], r[1], r[2], r[4], r[5], r[6],
+                               bits_f(refOf(L[i])), bits_f(refOf(R[i]))};
+        for (int k = 0; k < 14; ++k) o[k] = rec[k];
     }
     int32_t rootRef = refOf(0);
     // depth
@@ -506,12 +506,15 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
         }
     }
 
-    std::vector<float4> pos((size_t)nTris * 3), shade((size_t)nTris * 3);
+    // triangle tests read v0, e1 = v1 - v0, e2 = v2 - v0 packed in 36 bytes: more
+    // triangles per cache line than 16-byte-aligned rows would give
+    std::vector<float> pos((size_t)nTris * 9);
+    std::vector<float4> shade((size_t)nTris * 3);
     for (int i = 0; i < nTris; ++i) {
         const float *t = tris + 32 * (size_t)i;
-        pos[3 * i + 0] = make_float4(t[0], t[1], t[2], 0.0f);
-        pos[3 * i + 1] = make_float4(t[4] - t[0], t[5] - t[1], t[6] - t[2], 0.0f);
-        pos[3 * i + 2] = make_float4(t[8] - t[0], t[9] - t[1], t[10] - t[2], 0.0f);
+        const float rec[9] = {t[0], t[1], t[2], t[4] - t[0], t[5] - t[1], t[6] - t[2],
+                              t[8] - t[0], t[9] - t[1], t[10] - t[2]};
+        for (int k = 0; k < 9; ++k) pos[9 * (size_t)i + k] = rec[k];
         shade[3 * i + 0] = make_float4(t[12], t[13], t[14], t[30]);
         shade[3 * i + 1] = make_float4(t[16], t[17], t[18], 0.0f);
         shade[3 * i + 2] = make_float4(t[20], t[21], t[22], 0.0f);
@@ -529,15 +532,15 @@ static int upload_scene_into(rc_ctx *ctx, DeviceScene &D, const float *tris, int
     ++ctx->freeGen;
     free_scene(D);
     int rc;
-    if ((rc = dalloc(ctx, D.inner, inner.size() ? inner.size() : 4, "hipMalloc(nodes)"))) return rc;
+    if ((rc = dalloc(ctx, D.inner, inner.size() ? inner.size() : 14, "hipMalloc(nodes)"))) return rc;
     if ((rc = dalloc(ctx, D.triPos, pos.size(), "hipMalloc(triangles)"))) return rc;
     if ((rc = dalloc(ctx, D.triShade, shade.size(), "hipMalloc(shading)"))) return rc;
     if ((rc = dalloc(ctx, D.mats, (size_t)nMats * 4, "hipMalloc(materials)"))) return rc;
     if ((rc = dalloc(ctx, D.lights, (size_t)(nLights > 0 ? nLights : 1) * 5, "hipMalloc(lights)"))) return rc;
     if ((rc = dalloc(ctx, D.cdf, cdf.size(), "hipMalloc(cdf)"))) return rc;
     if ((rc = dalloc(ctx, D.leafBigCount, (size_t)nTris, "hipMalloc(leafBigCount)"))) return rc;
-    if ((rc = upload(ctx, D.inner, inner.data(), inner.size() * sizeof(float4), "upload(nodes)"))) return rc;
-    if ((rc = upload(ctx, D.triPos, pos.data(), pos.size() * sizeof(float4), "upload(triangles)"))) return rc;
+    if ((rc = upload(ctx, D.inner, inner.data(), inner.size() * sizeof(float), "upload(nodes)"))) return rc;
+    if ((rc = upload(ctx, D.triPos, pos.data(), pos.size() * sizeof(float), "upload(triangles)"))) return rc;
     if ((rc = upload(ctx, D.triShade, shade.data(), shade.size() * sizeof(float4), "upload(shading)"))) return rc;
     if ((rc = upload(ctx, D.mats, mats, (size_t)nMats * 64, "upload(materials)"))) return rc;
     if (nLights > 0 && (rc = upload(ctx, D.lights, lights, (size_t)nLights * 80, "upload(lights)"))) return rc;

@@ -62,8 +62,8 @@ __device__ __forceinline__ uint32_t init_seed(int x, int y, int frame) {
 }
 
 struct SceneView {
-    const float4 *__restrict__ inner;    // 4 per inner node
-    const float4 *__restrict__ triPos;   // 3 per triangle
+    const float *__restrict__ inner;     // 14 per inner node: left box, right box, left ref, right ref
+    const float *__restrict__ triPos;    // 9 per triangle: v0, e1 = v1 - v0, e2 = v2 - v0 (36 B, packed)
     const float4 *__restrict__ triShade; // 3 per triangle
     const float4 *__restrict__ mats;     // 4 per material
     const float4 *__restrict__ lights;   // 5 per light
@@ -115,9 +115,8 @@ __device__ __forceinline__ float recip_rn(float x) {
 // intersectTriangle (pathtrace.comp:112-130) on pre-subtracted edges.
 __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int idx,
                                          float &t, float &bu, float &bv) {
-    const float4 *p = S.triPos + 3 * (size_t)idx;
-    float4 a = p[0], b = p[1], c = p[2];
-    v3 v0 = xyz(a), e1 = xyz(b), e2 = xyz(c);
+    const float *p = S.triPos + 9 * (size_t)idx;
+    v3 v0 = mk(p[0], p[1], p[2]), e1 = mk(p[3], p[4], p[5]), e2 = mk(p[6], p[7], p[8]);
     v3 s = ro - v0;
     // s is formed here, before the det test, so that v0 is loaded together with
     // e1 and e2: left to itself the compiler moves v0's load below the det test,
@@ -136,6 +135,23 @@ __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int i
     if (tt < 1e-4f) return false;
     t = tt; bu = u; bv = v;
     return true;
+}
+
+// Inner node record, 14 words (56 B, packed): the left child's box (min xyz,
+// max xyz), the right child's, then the two child refs (>= 0: inner node index,
+// < 0: ~(first triangle << 5 | count)).
+struct NodeRec {
+    float4 a, b, c;  // a.xyz left min, a.w b.xy left max, b.zw c.x right min, c.yzw right max
+    int2 r;          // left ref, right ref
+};
+__device__ __forceinline__ NodeRec node_rec(const SceneView &S, int ref) {
+    const float *p = S.inner + 14 * (size_t)ref;
+    NodeRec n;
+    n.a = make_float4(p[0], p[1], p[2], p[3]);
+    n.b = make_float4(p[4], p[5], p[6], p[7]);
+    n.c = make_float4(p[8], p[9], p[10], p[11]);
+    n.r = make_int2(__float_as_int(p[12]), __float_as_int(p[13]));
+    return n;
 }
 
 __device__ __forceinline__ void leaf_range(const SceneView &S, int ref, int &start, int &count) {
@@ -206,9 +222,9 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &S, v3 ro, v3 rd, Sta
                 }
             }
         } else if (top < 62) {
-            const float4 *n = S.inner + 4 * (size_t)ref;
-            float4 a = n[0], b = n[1], c = n[2];
-            int4 r = ((const int4 *)n)[3];
+            const NodeRec nd = node_rec(S, ref);
+            const float4 a = nd.a, b = nd.b, c = nd.c;
+            const int2 r = nd.r;
             if (STATS) C.childTests++;
             float tl, trr;
             bool hl = aabb(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
@@ -243,16 +259,9 @@ __device__ __forceinline__ int entry_box(const SceneView &S, int e, float &mnx, 
         mxx = S.rootMax[0]; mxy = S.rootMax[1]; mxz = S.rootMax[2];
         return S.rootRef;
     }
-    const float4 *n = S.inner + 4 * (size_t)(e >> 1);
-    int4 r = ((const int4 *)n)[3];
-    if ((e & 1) == 0) {
-        float4 a = n[0], b = n[1];
-        mnx = a.x; mny = a.y; mnz = a.z; mxx = a.w; mxy = b.x; mxz = b.y;
-        return r.x;
-    }
-    float4 b = n[1], c = n[2];
-    mnx = b.z; mny = b.w; mnz = c.x; mxx = c.y; mxy = c.z; mxz = c.w;
-    return r.y;
+    const float *n = S.inner + 14 * (size_t)(e >> 1) + 6 * (e & 1);
+    mnx = n[0]; mny = n[1]; mnz = n[2]; mxx = n[3]; mxy = n[4]; mxz = n[5];
+    return __float_as_int(S.inner[14 * (size_t)(e >> 1) + 12 + (e & 1)]);
 }
 
 // Closest hit of the photon shaders (photon_trace.comp:101-124,
@@ -327,17 +336,9 @@ __device__ __forceinline__ bool unoccluded(const SceneView &S, v3 ro, v3 rd, flo
             mxx = S.rootMax[0]; mxy = S.rootMax[1]; mxz = S.rootMax[2];
             ref = S.rootRef;
         } else {
-            const float4 *n = S.inner + 4 * (size_t)(e >> 1);
-            int4 r = ((const int4 *)n)[3];
-            if ((e & 1) == 0) {
-                float4 a = n[0], b = n[1];
-                mnx = a.x; mny = a.y; mnz = a.z; mxx = a.w; mxy = b.x; mxz = b.y;
-                ref = r.x;
-            } else {
-                float4 b = n[1], c = n[2];
-                mnx = b.z; mny = b.w; mnz = c.x; mxx = c.y; mxy = c.z; mxz = c.w;
-                ref = r.y;
-            }
+            const float *n = S.inner + 14 * (size_t)(e >> 1) + 6 * (e & 1);
+            mnx = n[0]; mny = n[1]; mnz = n[2]; mxx = n[3]; mxy = n[4]; mxz = n[5];
+            ref = __float_as_int(S.inner[14 * (size_t)(e >> 1) + 12 + (e & 1)]);
         }
         float at;
         if (!aabb(ro, inv, mnx, mny, mnz, mxx, mxy, mxz, at)) continue;

@@ -24,8 +24,8 @@ CamArgs cam_args(const rc_camera &cam);  // rc_pathtrace.hip
 
 // Device scene (see rc_device.h for the layout).
 struct DeviceScene {
-    float4 *inner = nullptr;
-    float4 *triPos = nullptr;
+    float *inner = nullptr;  // 14 floats per inner node (NodeRec)
+    float *triPos = nullptr;  // 9 floats per triangle: v0, e1, e2
     float4 *triShade = nullptr;
     float4 *mats = nullptr;
     float4 *lights = nullptr;

@@ -127,9 +127,9 @@ __device__ __forceinline__ bool walk_step(const SceneView &S, Walk &w, bool anyH
         return w.top == 0;
     }
     if (w.top < 62) {
-        const float4 *n = S.inner + 4 * (size_t)w.ref;
-        const float4 a = n[0], b = n[1], c = n[2];
-        const int4 r = ((const int4 *)n)[3];
+        const NodeRec nd = node_rec(S, w.ref);
+        const float4 a = nd.a, b = nd.b, c = nd.c;
+        const int2 r = nd.r;
         if (STATS) C.childTests++;
         float tl, trr;
         const bool hl = aabb(w.ro, w.inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
@@ -172,9 +172,9 @@ __device__ __forceinline__ bool walk_step_tri(const SceneView &S, Walk &w, bool 
         if (w.tEntry > w.ht) return w.top == 0;
         if (w.ref >= 0) {
             if (w.top < 62) {
-                const float4 *n = S.inner + 4 * (size_t)w.ref;
-                const float4 a = n[0], b = n[1], c = n[2];
-                const int4 r = ((const int4 *)n)[3];
+                const NodeRec nd = node_rec(S, w.ref);
+                const float4 a = nd.a, b = nd.b, c = nd.c;
+                const int2 r = nd.r;
                 if (STATS) C.childTests++;
                 float tl, trr;
                 const bool hl = aabb(w.ro, w.inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
@@ -258,9 +258,9 @@ __device__ __forceinline__ bool walk_step_dual(const SceneView &S, Walk &w, bool
                 w.li = start;
                 w.le = start + count;
             } else if (w.top < 62) {
-                const float4 *n = S.inner + 4 * (size_t)w.ref;
-                const float4 a = n[0], b = n[1], c = n[2];
-                const int4 r = ((const int4 *)n)[3];
+                const NodeRec nd = node_rec(S, w.ref);
+                const float4 a = nd.a, b = nd.b, c = nd.c;
+                const int2 r = nd.r;
                 if (STATS) C.childTests++;
                 float tl, trr;
                 const bool hl = aabb(w.ro, w.inv, a.x, a.y, a.z, a.w, b.x, b.y, tl);
