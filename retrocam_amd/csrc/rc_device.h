@@ -122,19 +122,21 @@ __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int i
     // e1 and e2: left to itself the compiler moves v0's load below the det test,
     // which costs every accepted determinant a second memory round trip.
     asm volatile("" : "+v"(s.x), "+v"(s.y), "+v"(s.z));
+    // Every value is computed for every lane and the reference's early exits are
+    // one predicate at the end (the same comparisons, NaN-for-NaN): within a wave
+    // some lane nearly always passes each test, so branching would only add
+    // exec-mask work. The values of a lane that fails are never used.
     v3 h = cross(rd, e2);
     float det = dot(e1, h);
-    if (fabsf(det) < 1e-8f) return false;
     float f = recip_rn(det);
     float u = f * dot(s, h);
-    if (u < 0.0f || u > 1.0f) return false;
     v3 q = cross(s, e1);
     float v = f * dot(rd, q);
-    if (v < 0.0f || u + v > 1.0f) return false;
     float tt = f * dot(e2, q);
-    if (tt < 1e-4f) return false;
+    const bool ok = !(fabsf(det) < 1e-8f) && !(u < 0.0f || u > 1.0f) && !(v < 0.0f || u + v > 1.0f) &&
+                    !(tt < 1e-4f);
     t = tt; bu = u; bv = v;
-    return true;
+    return ok;
 }
 
 // Inner node record, 14 words (56 B, packed): the left child's box (min xyz,
