@@ -118,9 +118,9 @@ __device__ __forceinline__ bool tri_test(const SceneView &S, v3 ro, v3 rd, int i
     const float *p = S.triPos + 9 * (size_t)idx;
     v3 v0 = mk(p[0], p[1], p[2]), e1 = mk(p[3], p[4], p[5]), e2 = mk(p[6], p[7], p[8]);
     v3 s = ro - v0;
-    // s is formed here, before the det test, so that v0 is loaded together with
-    // e1 and e2: left to itself the compiler moves v0's load below the det test,
-    // which costs every accepted determinant a second memory round trip.
+    // s is formed first and held by an empty asm, so that v0 is loaded together
+    // with e1 and e2: with the test's early exits the compiler had moved v0's load
+    // below the determinant test, a second memory round trip per tested triangle.
     asm volatile("" : "+v"(s.x), "+v"(s.y), "+v"(s.z));
     // Every value is computed for every lane and the reference's early exits are
     // one predicate at the end (the same comparisons, NaN-for-NaN): within a wave
